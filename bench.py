@@ -1,0 +1,253 @@
+"""Benchmark of the gfx950 DFT engine on the plain FP32 sweep workload.
+
+Workload (one step): complex-f32 and complex-f64 C2C forward transforms at N = 2^4..2^24 with each
+batch sized to 1 GiB of input (f32: 2^27 points, f64: 2^26 points), R2C dft_plan_real<float> at
+N = 2^24 (batch 16) and N = 30,720,000 (batch 8), and DCT-II dct_plan<float> at N = 65536
+(batch 1024). Inputs are i.i.d. standard Gaussian from a fixed seed, resident in HBM before timing.
+
+value = whole-step GFLOP/s with 5 N log2 N per C2C transform and 2.5 N log2 N per R2C / DCT
+transform. The line also carries the HBM roofline of the dominant kernel, the rel-RMS error of
+sampled outputs against FP64, and the reference engine timed on the host CPU on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--quick]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
+
+
+def sweep_configs(quick: bool):
+    cfg = []
+    top = 20 if quick else 24
+    for a in range(4, top + 1):
+        n = 1 << a
+        cfg.append(("c2c", "f32", n, (1 << 27) // n))
+    for a in range(4, top + 1):
+        n = 1 << a
+        cfg.append(("c2c", "f64", n, (1 << 26) // n))
+    if not quick:
+        cfg.append(("r2c", "f32", 1 << 24, 16))
+        cfg.append(("r2c", "f32", 30720000, 8))
+        cfg.append(("dct", "f32", 65536, 1024))
+    return cfg
+
+
+def flops(kind, n, batch):
+    f = 5.0 * n * math.log2(n) * batch
+    return f if kind == "c2c" else f / 2.0
+
+
+def alg_bytes(kind, dt, n, batch):
+    """Algorithmic HBM bytes: one read of the input and one write of the output."""
+    es = 4 if dt == "f32" else 8
+    if kind == "c2c":
+        return 2 * (2 * es) * n * batch
+    if kind == "r2c":
+        return (es * n + 2 * es * (n // 2 + 1)) * batch
+    return 2 * es * n * batch
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="sizes up to 2^20 only, C2C only")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding is a design note only",
+                          "n_gpus": args.gpus}))
+        return 2
+
+    import torch
+
+    import kfr_amd
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream()
+    cfgs = sweep_configs(args.quick)
+
+    # plans (excluded from timing)
+    plans = {}
+    for kind, dt, n, batch in cfgs:
+        key = (kind, dt, n)
+        if kind == "c2c":
+            plans[key] = kfr_amd.dft_plan(n, np.complex64 if dt == "f32" else np.complex128)
+        elif kind == "r2c":
+            plans[key] = kfr_amd.dft_plan_real(n, kfr_amd.CCs, np.float32)
+        else:
+            plans[key] = kfr_amd.dct_plan(n, np.float32)
+
+    # inputs resident in HBM: 1 GiB of Gaussian values, reinterpreted per configuration
+    g = torch.Generator(device=dev)
+    g.manual_seed(20240611)
+    in_bytes = 1 << 30
+    src32 = torch.randn(in_bytes // 4, generator=g, device=dev, dtype=torch.float32)
+    src64 = torch.randn(in_bytes // 8, generator=g, device=dev, dtype=torch.float64)
+    out_bytes = max((1 << 30), 8 * (30720000 // 2 + 1) * 8 + 64)
+    dst = torch.empty(out_bytes // 4, device=dev, dtype=torch.float32)
+
+    def run_cfg(c):
+        kind, dt, n, batch = c
+        p = plans[(kind, dt, n)]
+        src = src32 if dt == "f32" else src64
+        out = dst if dt == "f32" else dst.view(torch.float64)
+        if kind == "c2c":
+            p.execute_batch(out, src, batch, False, stream)
+        elif kind == "r2c":
+            p.execute_batch(out, src, batch, False, stream)
+        else:
+            p.execute_batch(out, src, batch, False, stream)
+
+    for _ in range(args.warmup):
+        for c in cfgs:
+            run_cfg(c)
+    torch.cuda.synchronize()
+
+    ev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in cfgs]
+          for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record(stream)
+    for s in range(args.steps):
+        for i, c in enumerate(cfgs):
+            ev[s][i][0].record(stream)
+            run_cfg(c)
+            ev[s][i][1].record(stream)
+    end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    total_ms = start.elapsed_time(end)
+    ms_per_step = total_ms / args.steps
+
+    per = []
+    for i, c in enumerate(cfgs):
+        ts = [ev[s][i][0].elapsed_time(ev[s][i][1]) for s in range(args.steps)]
+        t = float(np.median(ts)) * 1e-3
+        kind, dt, n, batch = c
+        per.append({"kind": kind, "dtype": dt, "n": n, "batch": batch, "ms": t * 1e3,
+                    "gflops": flops(kind, n, batch) / t * 1e-9,
+                    "gbs": alg_bytes(kind, dt, n, batch) / t * 1e-9,
+                    "frac": alg_bytes(kind, dt, n, batch) / t * 1e-9 / HBM_PEAK_GBS})
+    step_flops = sum(flops(c[0], c[2], c[3]) for c in cfgs)
+    value = step_flops * args.steps / (total_ms * 1e-3) * 1e-9
+
+    # rel-RMS against FP64 on sampled transforms
+    errs = {}
+    for c in cfgs:
+        kind, dt, n, batch = c
+        nsamp = max(1, min(batch, (1 << 18) // n))
+        run_cfg(c)
+        torch.cuda.synchronize()
+        if kind == "c2c":
+            cdt = np.complex64 if dt == "f32" else np.complex128
+            src = (src32 if dt == "f32" else src64)
+            x = src[: 2 * n * nsamp].cpu().numpy().view(cdt).reshape(nsamp, n)
+            out = dst if dt == "f32" else dst.view(torch.float64)
+            y = out[: 2 * n * nsamp].cpu().numpy().view(cdt).reshape(nsamp, n)
+            ref = np.fft.fft(x.astype(np.complex128), axis=-1)
+        elif kind == "r2c":
+            x = src32[: n * nsamp].cpu().numpy().reshape(nsamp, n)
+            cs = n // 2 + 1
+            y = dst[: 2 * cs * nsamp].cpu().numpy().view(np.complex64).reshape(nsamp, cs)
+            ref = np.fft.rfft(x.astype(np.float64), axis=-1)
+        else:
+            import scipy.fft
+            x = src32[: n * nsamp].cpu().numpy().reshape(nsamp, n)
+            y = dst[: n * nsamp].cpu().numpy().reshape(nsamp, n)
+            ref = scipy.fft.dct(x.astype(np.float64), type=2, axis=-1) / 2.0
+        e = max(float(np.linalg.norm(y[b] - ref[b]) / np.linalg.norm(ref[b])) for b in range(nsamp))
+        k = f"{kind}_{dt}"
+        errs[k] = max(errs.get(k, 0.0), e)
+
+    # dominant kernel: the single-launch LDS kernel of the complex-f32 sizes N <= 2^14 (one launch per
+    # size, 2 GiB algorithmic per launch) or the two four-step kernels, whichever takes longer
+    lds = [p for p in per if p["kind"] == "c2c" and p["dtype"] == "f32" and p["n"] <= (1 << 14)]
+    fs = [p for p in per if p["kind"] == "c2c" and p["dtype"] == "f32" and p["n"] > (1 << 14)]
+    lds_t = sum(p["ms"] for p in lds)
+    lds_bytes = sum(alg_bytes("c2c", "f32", p["n"], p["batch"]) for p in lds)
+    achieved = lds_bytes / (lds_t * 1e-3) * 1e-9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_fft<float,false,role_direct> (complex-f32 N=2^4..2^14, one launch per size, "
+                          "2 GiB algorithmic bytes per launch)"}
+    fs_t = sum(p["ms"] for p in fs)
+    fs_bytes = sum(alg_bytes("c2c", "f32", p["n"], p["batch"]) for p in fs)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(cfgs)
+
+    line = {
+        "metric": "batched C2C FFT GFLOP/s (5N*log2N) & % HBM roofline; rel-RMS err vs FP64",
+        "value": round(value, 1),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (i.i.d. standard Gaussian, torch generator seed 20240611, resident in HBM)",
+        "config": {"workload": "plain sweep: C2C c32+c64 N=2^4..2^24 at 1 GiB input per size"
+                               + ("" if args.quick else ", R2C f32 N=2^24 x16 and N=30720000 x8, DCT-II f32 N=65536 x1024"),
+                   "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "rel_rms_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs.items()},
+        "fourstep_c2c_f32_frac": round(fs_bytes / (fs_t * 1e-3) * 1e-9 / HBM_PEAK_GBS, 4) if fs_t else None,
+        "wall_s": round(wall, 3),
+        "per_config": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in p.items()} for p in per],
+    }
+    print(json.dumps(line))
+    return 0
+
+
+def cpu_baseline(cfgs):
+    """The reference engine (oracle/_ref, built from the reference sources) on the host cores, on a
+    bounded sample of the complex-f32 sweep: each size runs a reduced batch, split across threads."""
+    try:
+        import oracle
+    except ImportError:
+        return None
+    ref = oracle.reference()
+    if ref is None:
+        return {"value": None, "unit": "GFLOP/s", "cores": 0, "kind": "reference",
+                "sample": "skipped: oracle/_ref not built"}
+    threads = max(1, min(16, os.cpu_count() or 1))
+    tot_f = 0.0
+    tot_t = 0.0
+    sizes = []
+    for kind, dt, n, batch in cfgs:
+        if kind != "c2c" or dt != "f32":
+            continue
+        b = max(threads, min(batch, (1 << 22) // n))
+        x = oracle.complex_gaussian(11, (b, n), np.complex64)
+        out = np.empty_like(x)
+        t = ref.time(0, 0, n, b, False, threads, 3, out, x)
+        tot_f += flops("c2c", n, b)
+        tot_t += t
+        sizes.append(f"{n}x{b}")
+    return {"value": round(tot_f / tot_t * 1e-9, 2), "unit": "GFLOP/s", "cores": threads, "kind": "reference",
+            "arch": ref.version(),
+            "sample": "complex-f32 forward, N=2^4..2^24 with 2^22 points per size (" + ", ".join(sizes) +
+                      "), median of 3 runs per size, batch split over threads"}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,256 @@
+/*
+ * C ABI of the MI355X (gfx950) DFT engine.
+ *
+ * Part 1 is a drop-in for the reference's C API (include/kfr/capi.h): the same function names,
+ * argument meanings, opaque plan types, pack-format enum and error channel, so a program written
+ * against the reference links against libkfr_amd.so unchanged. Each declaration names the
+ * reference declaration it replaces. Semantics kept from the reference:
+ *   - plans are created by *_create_* and freed by *_delete_plan_*; NULL is returned on failure and
+ *     kfr_last_error() describes why; every successful call clears the error (src/capi/capi.cpp:36-91);
+ *   - execute functions take host pointers (device pointers are also accepted), run synchronously,
+ *     apply no scaling, and allow out == in; `temp` may be NULL (the GPU keeps its own scratch);
+ *   - *_get_temp_size returns the same byte count the reference plan reports.
+ *
+ * Part 2 holds extensions the reference does not have: batched transforms on device pointers with
+ * a HIP stream, FFT convolution with its adjoint, plan descriptors and the GPU schedule.
+ */
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C"
+{
+#endif
+
+#define KFR_AMD_API __attribute__((visibility("default")))
+
+/* include/kfr/capi.h:77-96 */
+#if defined(__cplusplus)
+typedef bool kfr_bool;
+#else
+typedef _Bool kfr_bool;
+#endif
+
+/* include/kfr/capi.h:98-111 */
+enum
+{
+    KFR_ARCH_X86    = 0,
+    KFR_ARCH_SSE2   = 1,
+    KFR_ARCH_SSE3   = 2,
+    KFR_ARCH_SSSE3  = 3,
+    KFR_ARCH_SSE41  = 4,
+    KFR_ARCH_SSE42  = 5,
+    KFR_ARCH_AVX    = 6,
+    KFR_ARCH_AVX2   = 7,
+    KFR_ARCH_AVX512 = 8,
+    KFR_ARCH_GFX950 = 100 /* this library's only target */
+};
+
+#define KFR_HEADERS_VERSION 70000 /* include/kfr/capi.h:113 */
+
+/* include/kfr/capi.h:131-142: complex values are interleaved (re, im) pairs */
+typedef float kfr_f32;
+typedef double kfr_f64;
+typedef float kfr_c32;
+typedef double kfr_c64;
+#define KFR_COMPLEX_SIZE_MULTIPLIER 2
+
+/* include/kfr/capi.h:145-164 */
+typedef struct KFR_DFT_PLAN_F32 { int opaque; } KFR_DFT_PLAN_F32;
+typedef struct KFR_DFT_PLAN_F64 { int opaque; } KFR_DFT_PLAN_F64;
+typedef struct KFR_DFT_REAL_PLAN_F32 { int opaque; } KFR_DFT_REAL_PLAN_F32;
+typedef struct KFR_DFT_REAL_PLAN_F64 { int opaque; } KFR_DFT_REAL_PLAN_F64;
+typedef struct KFR_DCT_PLAN_F32 { int opaque; } KFR_DCT_PLAN_F32;
+typedef struct KFR_DCT_PLAN_F64 { int opaque; } KFR_DCT_PLAN_F64;
+typedef struct KFR_FILTER_F32 { int opaque; } KFR_FILTER_F32;
+typedef struct KFR_FILTER_F64 { int opaque; } KFR_FILTER_F64;
+typedef struct KFR_FILTER_C32 { int opaque; } KFR_FILTER_C32;
+typedef struct KFR_FILTER_C64 { int opaque; } KFR_FILTER_C64;
+
+#define KFR_DEFAULT_ALIGNMENT 64 /* include/kfr/capi.h:167 */
+
+/* include/kfr/capi.h:196-200 */
+typedef enum KFR_DFT_PACK_FORMAT
+{
+    Perm = 0,
+    CCs  = 1
+} KFR_DFT_PACK_FORMAT;
+
+/* ============================== Part 1: drop-in ============================== */
+
+/* library information: include/kfr/capi.h:116-128 */
+KFR_AMD_API const char* kfr_version_string(void);
+KFR_AMD_API uint32_t kfr_version(void);
+KFR_AMD_API const char* kfr_enabled_archs(void);
+KFR_AMD_API int kfr_current_arch(void);
+KFR_AMD_API const char* kfr_last_error(void);
+
+/* host memory: include/kfr/capi.h:170-176 */
+KFR_AMD_API void* kfr_allocate(size_t size);
+KFR_AMD_API void* kfr_allocate_aligned(size_t size, size_t alignment);
+KFR_AMD_API void kfr_deallocate(void* ptr);
+
+/* complex DFT: include/kfr/capi.h:207-372 */
+KFR_AMD_API KFR_DFT_PLAN_F32* kfr_dft_create_plan_f32(size_t size);
+KFR_AMD_API KFR_DFT_PLAN_F64* kfr_dft_create_plan_f64(size_t size);
+KFR_AMD_API KFR_DFT_PLAN_F32* kfr_dft_create_2d_plan_f32(size_t size1, size_t size2);
+KFR_AMD_API KFR_DFT_PLAN_F64* kfr_dft_create_2d_plan_f64(size_t size1, size_t size2);
+KFR_AMD_API KFR_DFT_PLAN_F32* kfr_dft_create_3d_plan_f32(size_t size1, size_t size2, size_t size3);
+KFR_AMD_API KFR_DFT_PLAN_F64* kfr_dft_create_3d_plan_f64(size_t size1, size_t size2, size_t size3);
+KFR_AMD_API KFR_DFT_PLAN_F32* kfr_dft_create_md_plan_f32(size_t dims, const unsigned* shape);
+KFR_AMD_API KFR_DFT_PLAN_F64* kfr_dft_create_md_plan_f64(size_t dims, const unsigned* shape);
+KFR_AMD_API void kfr_dft_dump_f32(KFR_DFT_PLAN_F32* plan);
+KFR_AMD_API void kfr_dft_dump_f64(KFR_DFT_PLAN_F64* plan);
+KFR_AMD_API size_t kfr_dft_get_size_f32(KFR_DFT_PLAN_F32* plan);
+KFR_AMD_API size_t kfr_dft_get_size_f64(KFR_DFT_PLAN_F64* plan);
+KFR_AMD_API size_t kfr_dft_get_temp_size_f32(KFR_DFT_PLAN_F32* plan);
+KFR_AMD_API size_t kfr_dft_get_temp_size_f64(KFR_DFT_PLAN_F64* plan);
+KFR_AMD_API void kfr_dft_execute_f32(KFR_DFT_PLAN_F32* plan, kfr_c32* out, const kfr_c32* in, uint8_t* temp);
+KFR_AMD_API void kfr_dft_execute_f64(KFR_DFT_PLAN_F64* plan, kfr_c64* out, const kfr_c64* in, uint8_t* temp);
+KFR_AMD_API void kfr_dft_execute_inverse_f32(KFR_DFT_PLAN_F32* plan, kfr_c32* out, const kfr_c32* in,
+                                             uint8_t* temp);
+KFR_AMD_API void kfr_dft_execute_inverse_f64(KFR_DFT_PLAN_F64* plan, kfr_c64* out, const kfr_c64* in,
+                                             uint8_t* temp);
+KFR_AMD_API void kfr_dft_delete_plan_f32(KFR_DFT_PLAN_F32* plan);
+KFR_AMD_API void kfr_dft_delete_plan_f64(KFR_DFT_PLAN_F64* plan);
+
+/* real DFT: include/kfr/capi.h:380-516 */
+KFR_AMD_API KFR_DFT_REAL_PLAN_F32* kfr_dft_real_create_plan_f32(size_t size, KFR_DFT_PACK_FORMAT pack_format);
+KFR_AMD_API KFR_DFT_REAL_PLAN_F64* kfr_dft_real_create_plan_f64(size_t size, KFR_DFT_PACK_FORMAT pack_format);
+KFR_AMD_API KFR_DFT_REAL_PLAN_F32* kfr_dft_real_create_2d_plan_f32(size_t size1, size_t size2,
+                                                                   kfr_bool real_out_is_enough);
+KFR_AMD_API KFR_DFT_REAL_PLAN_F64* kfr_dft_real_create_2d_plan_f64(size_t size1, size_t size2,
+                                                                   kfr_bool real_out_is_enough);
+KFR_AMD_API KFR_DFT_REAL_PLAN_F32* kfr_dft_real_create_3d_plan_f32(size_t size1, size_t size2, size_t size3,
+                                                                   kfr_bool real_out_is_enough);
+KFR_AMD_API KFR_DFT_REAL_PLAN_F64* kfr_dft_real_create_3d_plan_f64(size_t size1, size_t size2, size_t size3,
+                                                                   kfr_bool real_out_is_enough);
+KFR_AMD_API KFR_DFT_REAL_PLAN_F32* kfr_dft_real_create_md_plan_f32(size_t dims, const unsigned* shape,
+                                                                   kfr_bool real_out_is_enough);
+KFR_AMD_API KFR_DFT_REAL_PLAN_F64* kfr_dft_real_create_md_plan_f64(size_t dims, const unsigned* shape,
+                                                                   kfr_bool real_out_is_enough);
+KFR_AMD_API void kfr_dft_real_dump_f32(KFR_DFT_REAL_PLAN_F32* plan);
+KFR_AMD_API void kfr_dft_real_dump_f64(KFR_DFT_REAL_PLAN_F64* plan);
+KFR_AMD_API size_t kfr_dft_real_get_size_f32(KFR_DFT_REAL_PLAN_F32* plan);
+KFR_AMD_API size_t kfr_dft_real_get_size_f64(KFR_DFT_REAL_PLAN_F64* plan);
+KFR_AMD_API size_t kfr_dft_real_get_temp_size_f32(KFR_DFT_REAL_PLAN_F32* plan);
+KFR_AMD_API size_t kfr_dft_real_get_temp_size_f64(KFR_DFT_REAL_PLAN_F64* plan);
+KFR_AMD_API void kfr_dft_real_execute_f32(KFR_DFT_REAL_PLAN_F32* plan, kfr_c32* out, const kfr_f32* in,
+                                          uint8_t* temp);
+KFR_AMD_API void kfr_dft_real_execute_f64(KFR_DFT_REAL_PLAN_F64* plan, kfr_c64* out, const kfr_f64* in,
+                                          uint8_t* temp);
+KFR_AMD_API void kfr_dft_real_execute_inverse_f32(KFR_DFT_REAL_PLAN_F32* plan, kfr_f32* out, const kfr_c32* in,
+                                                  uint8_t* temp);
+KFR_AMD_API void kfr_dft_real_execute_inverse_f64(KFR_DFT_REAL_PLAN_F64* plan, kfr_f64* out, const kfr_c64* in,
+                                                  uint8_t* temp);
+KFR_AMD_API void kfr_dft_real_delete_plan_f32(KFR_DFT_REAL_PLAN_F32* plan);
+KFR_AMD_API void kfr_dft_real_delete_plan_f64(KFR_DFT_REAL_PLAN_F64* plan);
+
+/* DCT-II / DCT-III: include/kfr/capi.h:523-636 */
+KFR_AMD_API KFR_DCT_PLAN_F32* kfr_dct_create_plan_f32(size_t size);
+KFR_AMD_API KFR_DCT_PLAN_F64* kfr_dct_create_plan_f64(size_t size);
+KFR_AMD_API void kfr_dct_dump_f32(KFR_DCT_PLAN_F32* plan);
+KFR_AMD_API void kfr_dct_dump_f64(KFR_DCT_PLAN_F64* plan);
+KFR_AMD_API size_t kfr_dct_get_size_f32(KFR_DCT_PLAN_F32* plan);
+KFR_AMD_API size_t kfr_dct_get_size_f64(KFR_DCT_PLAN_F64* plan);
+KFR_AMD_API size_t kfr_dct_get_temp_size_f32(KFR_DCT_PLAN_F32* plan);
+KFR_AMD_API size_t kfr_dct_get_temp_size_f64(KFR_DCT_PLAN_F64* plan);
+KFR_AMD_API void kfr_dct_execute_f32(KFR_DCT_PLAN_F32* plan, kfr_f32* out, const kfr_f32* in, uint8_t* temp);
+KFR_AMD_API void kfr_dct_execute_f64(KFR_DCT_PLAN_F64* plan, kfr_f64* out, const kfr_f64* in, uint8_t* temp);
+KFR_AMD_API void kfr_dct_execute_inverse_f32(KFR_DCT_PLAN_F32* plan, kfr_f32* out, const kfr_f32* in,
+                                             uint8_t* temp);
+KFR_AMD_API void kfr_dct_execute_inverse_f64(KFR_DCT_PLAN_F64* plan, kfr_f64* out, const kfr_f64* in,
+                                             uint8_t* temp);
+KFR_AMD_API void kfr_dct_delete_plan_f32(KFR_DCT_PLAN_F32* plan);
+KFR_AMD_API void kfr_dct_delete_plan_f64(KFR_DCT_PLAN_F64* plan);
+
+/* filters: include/kfr/capi.h:644-736. Only the convolution filter (convolve_filter) is on the DFT
+ * path; the FIR and IIR constructors return NULL with an error naming them as not provided. */
+KFR_AMD_API KFR_FILTER_F32* kfr_filter_create_fir_plan_f32(const kfr_f32* taps, size_t size);
+KFR_AMD_API KFR_FILTER_F64* kfr_filter_create_fir_plan_f64(const kfr_f64* taps, size_t size);
+KFR_AMD_API KFR_FILTER_F32* kfr_filter_create_convolution_plan_f32(const kfr_f32* taps, size_t size,
+                                                                   size_t block_size);
+KFR_AMD_API KFR_FILTER_F64* kfr_filter_create_convolution_plan_f64(const kfr_f64* taps, size_t size,
+                                                                   size_t block_size);
+KFR_AMD_API KFR_FILTER_F32* kfr_filter_create_iir_plan_f32(const kfr_f32* sos, size_t sos_count);
+KFR_AMD_API KFR_FILTER_F64* kfr_filter_create_iir_plan_f64(const kfr_f64* sos, size_t sos_count);
+KFR_AMD_API void kfr_filter_process_f32(KFR_FILTER_F32* plan, kfr_f32* output, const kfr_f32* input, size_t size);
+KFR_AMD_API void kfr_filter_process_f64(KFR_FILTER_F64* plan, kfr_f64* output, const kfr_f64* input, size_t size);
+KFR_AMD_API void kfr_filter_reset_f32(KFR_FILTER_F32* plan);
+KFR_AMD_API void kfr_filter_reset_f64(KFR_FILTER_F64* plan);
+KFR_AMD_API void kfr_filter_delete_plan_f32(KFR_FILTER_F32* plan);
+KFR_AMD_API void kfr_filter_delete_plan_f64(KFR_FILTER_F64* plan);
+
+/* ============================== Part 2: extensions ============================== */
+
+/* Batched transforms on device pointers, enqueued on `stream` (a hipStream_t; NULL = the library's
+ * stream) and not synchronised. Transforms are contiguous: item b starts at b*size (complex),
+ * b*size reals / b*complex_size complex values (real), b*size reals (DCT). Returns 0 on success,
+ * -1 on error (see kfr_last_error). */
+KFR_AMD_API int kfr_amd_dft_execute_batch_f32(KFR_DFT_PLAN_F32* plan, kfr_c32* out, const kfr_c32* in,
+                                              size_t batch, int inverse, void* stream);
+KFR_AMD_API int kfr_amd_dft_execute_batch_f64(KFR_DFT_PLAN_F64* plan, kfr_c64* out, const kfr_c64* in,
+                                              size_t batch, int inverse, void* stream);
+KFR_AMD_API int kfr_amd_dft_real_execute_batch_f32(KFR_DFT_REAL_PLAN_F32* plan, void* out, const void* in,
+                                                   size_t batch, int inverse, void* stream);
+KFR_AMD_API int kfr_amd_dft_real_execute_batch_f64(KFR_DFT_REAL_PLAN_F64* plan, void* out, const void* in,
+                                                   size_t batch, int inverse, void* stream);
+KFR_AMD_API int kfr_amd_dct_execute_batch_f32(KFR_DCT_PLAN_F32* plan, kfr_f32* out, const kfr_f32* in,
+                                              size_t batch, int inverse, void* stream);
+KFR_AMD_API int kfr_amd_dct_execute_batch_f64(KFR_DCT_PLAN_F64* plan, kfr_f64* out, const kfr_f64* in,
+                                              size_t batch, int inverse, void* stream);
+
+/* Complex size of a real plan (size/2+1 for CCs, size/2 for Perm): dft_plan_real::complex_size,
+ * include/kfr/dft/fft.hpp:471-476. */
+KFR_AMD_API size_t kfr_amd_dft_real_get_complex_size_f32(KFR_DFT_REAL_PLAN_F32* plan);
+KFR_AMD_API size_t kfr_amd_dft_real_get_complex_size_f64(KFR_DFT_REAL_PLAN_F64* plan);
+
+/* convolve / correlate / autocorrelate (include/kfr/dft/convolution.hpp:53-76) of host or device
+ * vectors. dtype: 0 f32, 1 f64, 2 c32, 3 c64; mode: 0 convolve, 1 correlate, 2 autocorrelate (b is
+ * ignored). Writes the result to `out` when it is not NULL and returns its length, or -1 on error.
+ * Complex inputs keep only the real part of the result, as the reference does. */
+KFR_AMD_API long kfr_amd_convolve(int dtype, int mode, const void* a, size_t na, const void* b, size_t nb,
+                                  void* out);
+
+/* Batched real FFT convolution on device pointers with a hand-written adjoint.
+ * y[b] = x[b] (*) h[b], |y| = nx + nh - 1; h_dist = 0 shares one filter across the batch.
+ * backward: dx = g correlated with h (first nx), dh = g correlated with x (first nh, summed over the
+ * batch when h_dist = 0); the signal spectrum is recomputed. dx or dh may be NULL. */
+typedef struct KFR_AMD_CONV { int opaque; } KFR_AMD_CONV;
+KFR_AMD_API KFR_AMD_CONV* kfr_amd_conv_create(int dtype, size_t nx, size_t nh);
+KFR_AMD_API size_t kfr_amd_conv_fft_size(KFR_AMD_CONV* conv);
+KFR_AMD_API int kfr_amd_conv_forward(KFR_AMD_CONV* conv, void* y, const void* x, const void* h, size_t h_dist,
+                                     size_t batch, int correlate, void* stream);
+KFR_AMD_API int kfr_amd_conv_backward(KFR_AMD_CONV* conv, void* dx, void* dh, const void* g, const void* x,
+                                      const void* h, size_t h_dist, size_t batch, void* stream);
+KFR_AMD_API void kfr_amd_conv_delete(KFR_AMD_CONV* conv);
+
+/* The reference plan descriptor for a size, computed on the host without a GPU: text in the layout of
+ * dft_plan::dump() (one line per stage) preceded by a summary line. kind: 0 dft_plan, 1 dft_plan_real
+ * (arg = pack format), 2 dct_plan, 3 dft_plan with progressive_optimized. Returns the text length
+ * (the buffer receives at most buflen-1 bytes) or -1. */
+KFR_AMD_API int kfr_amd_describe(int kind, int dtype, size_t size, int arg, char* buf, size_t buflen);
+
+/* The permutation applied by the reference plan's reorder stage (out[i] = in[perm[i]]): radix-2 bit
+ * reversal or mixed-radix digit reversal. Writes `size` indexes; returns 1, or 0 if the plan has no
+ * reorder stage, or -1 on error. */
+KFR_AMD_API int kfr_amd_dft_permutation(int dtype, size_t size, int64_t* perm);
+
+/* Number of stages of the reference plan, i.e. dft_plan::progressive_total_steps()
+ * (include/kfr/dft/fft.hpp:389-391). */
+KFR_AMD_API size_t kfr_amd_dft_progressive_total_steps(int dtype, size_t size);
+
+/* The GPU schedule chosen for a complex DFT size (algorithm, radices, launch geometry). */
+KFR_AMD_API int kfr_amd_dft_schedule(int dtype, size_t size, char* buf, size_t buflen);
+
+/* Test hook: the n-th following device allocation made by the library fails (0 disables). */
+KFR_AMD_API void kfr_amd_inject_alloc_failure(int n);
+
+/* Synchronise the library's stream. */
+KFR_AMD_API int kfr_amd_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif

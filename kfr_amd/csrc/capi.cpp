@@ -1,0 +1,892 @@
+// C ABI of the gfx950 DFT engine: the reference's kfr_dft_* / kfr_dft_real_* / kfr_dct_* /
+// kfr_filter_* entry points (src/capi/capi.cpp) over GPU plans, plus the device extensions.
+#include "../../include/kfr_amd_capi.h"
+#include "fft_engine.hpp"
+#include "plan_desc.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+
+using namespace kfr_amd;
+
+namespace
+{
+
+// thread-local error text, cleared by every successful call (src/capi/capi.cpp:36-91)
+thread_local std::array<char, 256> latest_error{};
+
+void reset_error() { latest_error.fill(0); }
+void set_error(const std::string& s)
+{
+    const size_t n = std::min(s.size(), latest_error.size() - 1);
+    std::copy_n(s.begin(), n, latest_error.begin());
+    std::fill(latest_error.begin() + n, latest_error.end(), 0);
+}
+
+template <typename Fn, typename R>
+R try_fn(Fn&& fn, R fallback)
+{
+    try
+    {
+        R r = fn();
+        reset_error();
+        return r;
+    }
+    catch (const std::exception& e)
+    {
+        set_error(e.what());
+        return fallback;
+    }
+    catch (...)
+    {
+        set_error("(unknown exception)");
+        return fallback;
+    }
+}
+template <typename Fn>
+void try_void(Fn&& fn)
+{
+    try_fn(
+        [&]
+        {
+            fn();
+            return 0;
+        },
+        0);
+}
+
+void check_plan(const void* p)
+{
+    if (!p)
+        throw std::invalid_argument("plan is NULL");
+}
+
+bool is_device_ptr(const void* p)
+{
+    if (!p)
+        return false;
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged || a.type == hipMemoryTypeUnified;
+}
+
+hipStream_t as_stream(void* s) { return s ? static_cast<hipStream_t>(s) : library_stream(); }
+
+// Runs fn(dev_out, dev_in) on device buffers: the user's pointers when both are device memory,
+// otherwise one staging buffer (the transforms are safe in place for a single item) with copies.
+struct stager
+{
+    dev_mem buf;
+    std::mutex mu;
+    template <typename Fn>
+    void run(void* out, size_t out_bytes, const void* in, size_t in_bytes, Fn&& fn)
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        hipStream_t s = library_stream();
+        if (is_device_ptr(in) && is_device_ptr(out))
+        {
+            fn(out, in, s);
+            hip_check(hipStreamSynchronize(s), "execute");
+            return;
+        }
+        buf.ensure(std::max(in_bytes, out_bytes));
+        hip_check(hipMemcpyAsync(buf.ptr, in, in_bytes, hipMemcpyDefault, s), "input copy");
+        fn(buf.ptr, buf.ptr, s);
+        hip_check(hipMemcpyAsync(out, buf.ptr, out_bytes, hipMemcpyDefault, s), "output copy");
+        hip_check(hipStreamSynchronize(s), "execute");
+    }
+};
+
+template <typename T>
+struct dft_obj
+{
+    plan_desc desc;
+    gpu_dft<T> g;
+    stager st;
+    explicit dft_obj(size_t n)
+        : desc(describe_dft(sizeof(T) == 4 ? dtype_t::f32 : dtype_t::f64, n, false)), g(n)
+    {
+    }
+};
+template <typename T>
+struct real_obj
+{
+    plan_desc desc;
+    gpu_dft_real<T> g;
+    stager st;
+    real_obj(size_t n, int fmt)
+        : desc(describe_dft_real(sizeof(T) == 4 ? dtype_t::f32 : dtype_t::f64, n, fmt)), g(n, fmt)
+    {
+    }
+};
+template <typename T>
+struct dct_obj
+{
+    plan_desc desc;
+    gpu_dct<T> g;
+    stager st;
+    explicit dct_obj(size_t n) : desc(describe_dct(sizeof(T) == 4 ? dtype_t::f32 : dtype_t::f64, n)), g(n) {}
+};
+template <typename T>
+struct filter_obj
+{
+    gpu_conv_filter<T> g;
+    dev_mem in_buf, out_buf;
+    std::mutex mu;
+    filter_obj(const T* taps, size_t n, size_t block) : g(taps, n, block) {}
+};
+
+void print_dump(const plan_desc& d, const schedule_info& s)
+{
+    std::fputs(dump_text(d, "gfx950").c_str(), stdout);
+    std::printf("%s\n", s.text().c_str());
+    std::fflush(stdout);
+}
+
+template <typename T>
+dft_obj<T>* dft_cast(void* p)
+{
+    check_plan(p);
+    return static_cast<dft_obj<T>*>(p);
+}
+template <typename T>
+real_obj<T>* real_cast(void* p)
+{
+    check_plan(p);
+    return static_cast<real_obj<T>*>(p);
+}
+template <typename T>
+dct_obj<T>* dct_cast(void* p)
+{
+    check_plan(p);
+    return static_cast<dct_obj<T>*>(p);
+}
+
+template <typename T>
+void dft_exec(void* plan, void* out, const void* in, bool inverse)
+{
+    auto* o        = dft_cast<T>(plan);
+    const size_t b = o->g.size() * 2 * sizeof(T);
+    o->st.run(out, b, in, b, [&](void* dout, const void* din, hipStream_t s) { o->g.execute(dout, din, 1, inverse, s); });
+}
+template <typename T>
+void real_exec(void* plan, void* out, const void* in, bool inverse)
+{
+    auto* o         = real_cast<T>(plan);
+    const size_t rb = o->g.size() * sizeof(T);
+    const size_t cb = o->g.complex_size() * 2 * sizeof(T);
+    if (!inverse)
+        o->st.run(out, cb, in, rb, [&](void* dout, const void* din, hipStream_t s) { o->g.forward(dout, din, 1, s); });
+    else
+        o->st.run(out, rb, in, cb, [&](void* dout, const void* din, hipStream_t s) { o->g.inverse(dout, din, 1, s); });
+}
+template <typename T>
+void dct_exec(void* plan, void* out, const void* in, bool inverse)
+{
+    auto* o        = dct_cast<T>(plan);
+    const size_t b = o->g.size() * sizeof(T);
+    o->st.run(out, b, in, b, [&](void* dout, const void* din, hipStream_t s) { o->g.execute(dout, din, 1, inverse, s); });
+}
+
+template <typename T>
+void filter_process(void* plan, T* out, const T* in, size_t n)
+{
+    check_plan(plan);
+    auto* f = static_cast<filter_obj<T>*>(plan);
+    std::lock_guard<std::mutex> lock(f->mu);
+    hipStream_t s = library_stream();
+    if (n == 0)
+        return;
+    if (is_device_ptr(in) && is_device_ptr(out) && in != out)
+    {
+        f->g.process(out, in, n, s);
+        hip_check(hipStreamSynchronize(s), "filter process");
+        return;
+    }
+    f->in_buf.ensure(n * sizeof(T));
+    f->out_buf.ensure(n * sizeof(T));
+    hip_check(hipMemcpyAsync(f->in_buf.ptr, in, n * sizeof(T), hipMemcpyDefault, s), "filter input copy");
+    f->g.process(f->out_buf.template as<T>(), f->in_buf.template as<T>(), n, s);
+    hip_check(hipMemcpyAsync(out, f->out_buf.ptr, n * sizeof(T), hipMemcpyDefault, s), "filter output copy");
+    hip_check(hipStreamSynchronize(s), "filter process");
+}
+
+[[noreturn]] void not_provided(const char* what)
+{
+    throw std::logic_error(std::string(what) + " is not provided by this library (DFT path only)");
+}
+
+template <typename T>
+long convolve_impl(int mode, const void* a, size_t na, const void* b, size_t nb, void* out, bool complex_in)
+{
+    if (mode == 2)
+    {
+        b  = a;
+        nb = na;
+    }
+    if (na == 0 || nb == 0)
+        throw std::invalid_argument("convolve: empty input");
+    const bool correlate = mode != 0;
+    const size_t ny      = na + nb - 1;
+    const size_t es      = complex_in ? 2 * sizeof(T) : sizeof(T);
+    // results of autocorrelate keep the second half (include/kfr/dft/convolution.hpp:70-76)
+    const size_t off  = mode == 2 ? ny / 2 : 0;
+    const size_t nout = ny - off;
+    if (!out)
+        return (long)nout;
+    hipStream_t s = library_stream();
+    dev_mem da(na * es), db(nb * es), dy(ny * es);
+    hip_check(hipMemcpyAsync(da.ptr, a, na * es, hipMemcpyDefault, s), "convolve input copy");
+    hip_check(hipMemcpyAsync(db.ptr, b, nb * es, hipMemcpyDefault, s), "convolve input copy");
+    if (complex_in)
+        gpu_convolve_complex<T>(dy.ptr, da.ptr, na, db.ptr, nb, correlate, false, s);
+    else
+    {
+        gpu_conv<T> c(na, nb);
+        c.forward(dy.ptr, da.ptr, db.ptr, 0, 1, correlate, s);
+    }
+    hip_check(hipMemcpyAsync(out, static_cast<char*>(dy.ptr) + off * es, nout * es, hipMemcpyDefault, s),
+              "convolve output copy");
+    hip_check(hipStreamSynchronize(s), "convolve");
+    return (long)nout;
+}
+
+struct conv_obj
+{
+    int dtype;
+    std::unique_ptr<gpu_conv<float>> f;
+    std::unique_ptr<gpu_conv<double>> d;
+    std::mutex mu;
+};
+
+int copy_text(const std::string& s, char* buf, size_t buflen)
+{
+    if (buf && buflen)
+    {
+        const size_t n = std::min(buflen - 1, s.size());
+        std::memcpy(buf, s.data(), n);
+        buf[n] = 0;
+    }
+    return (int)s.size();
+}
+
+dtype_t to_dtype(int d)
+{
+    if (d != 0 && d != 1)
+        throw std::invalid_argument("dtype must be 0 (f32) or 1 (f64)");
+    return d == 0 ? dtype_t::f32 : dtype_t::f64;
+}
+
+} // namespace
+
+extern "C"
+{
+
+    const char* kfr_version_string(void) { return "KFR 7.0.1 gfx950 64-bit (hip/MI355X) kfr_amd"; }
+    uint32_t kfr_version(void) { return 7 * 10000 + 0 * 100 + 1; }
+    const char* kfr_enabled_archs(void) { return "gfx950"; }
+    int kfr_current_arch(void) { return KFR_ARCH_GFX950; }
+    const char* kfr_last_error(void) { return latest_error.data(); }
+
+    void* kfr_allocate(size_t size) { return kfr_allocate_aligned(size, KFR_DEFAULT_ALIGNMENT); }
+    void* kfr_allocate_aligned(size_t size, size_t alignment)
+    {
+        // one header slot before the block keeps the base pointer for kfr_deallocate
+        if (alignment < sizeof(void*) || (alignment & (alignment - 1)))
+            alignment = KFR_DEFAULT_ALIGNMENT;
+        void* raw = std::malloc(size + alignment + sizeof(void*));
+        if (!raw)
+            return nullptr;
+        uintptr_t p = (reinterpret_cast<uintptr_t>(raw) + sizeof(void*) + alignment - 1) & ~(uintptr_t)(alignment - 1);
+        reinterpret_cast<void**>(p)[-1] = raw;
+        return reinterpret_cast<void*>(p);
+    }
+    void kfr_deallocate(void* ptr)
+    {
+        if (ptr)
+            std::free(reinterpret_cast<void**>(ptr)[-1]);
+    }
+
+    // ---- complex DFT
+    KFR_DFT_PLAN_F32* kfr_dft_create_plan_f32(size_t size)
+    {
+        return try_fn([&] { return reinterpret_cast<KFR_DFT_PLAN_F32*>(new dft_obj<float>(size)); },
+                      (KFR_DFT_PLAN_F32*)nullptr);
+    }
+    KFR_DFT_PLAN_F64* kfr_dft_create_plan_f64(size_t size)
+    {
+        return try_fn([&] { return reinterpret_cast<KFR_DFT_PLAN_F64*>(new dft_obj<double>(size)); },
+                      (KFR_DFT_PLAN_F64*)nullptr);
+    }
+    KFR_DFT_PLAN_F32* kfr_dft_create_2d_plan_f32(size_t, size_t)
+    {
+        return try_fn([&]() -> KFR_DFT_PLAN_F32* { not_provided("multidimensional DFT"); }, (KFR_DFT_PLAN_F32*)nullptr);
+    }
+    KFR_DFT_PLAN_F64* kfr_dft_create_2d_plan_f64(size_t, size_t)
+    {
+        return try_fn([&]() -> KFR_DFT_PLAN_F64* { not_provided("multidimensional DFT"); }, (KFR_DFT_PLAN_F64*)nullptr);
+    }
+    KFR_DFT_PLAN_F32* kfr_dft_create_3d_plan_f32(size_t, size_t, size_t)
+    {
+        return try_fn([&]() -> KFR_DFT_PLAN_F32* { not_provided("multidimensional DFT"); }, (KFR_DFT_PLAN_F32*)nullptr);
+    }
+    KFR_DFT_PLAN_F64* kfr_dft_create_3d_plan_f64(size_t, size_t, size_t)
+    {
+        return try_fn([&]() -> KFR_DFT_PLAN_F64* { not_provided("multidimensional DFT"); }, (KFR_DFT_PLAN_F64*)nullptr);
+    }
+    KFR_DFT_PLAN_F32* kfr_dft_create_md_plan_f32(size_t, const unsigned*)
+    {
+        return try_fn([&]() -> KFR_DFT_PLAN_F32* { not_provided("multidimensional DFT"); }, (KFR_DFT_PLAN_F32*)nullptr);
+    }
+    KFR_DFT_PLAN_F64* kfr_dft_create_md_plan_f64(size_t, const unsigned*)
+    {
+        return try_fn([&]() -> KFR_DFT_PLAN_F64* { not_provided("multidimensional DFT"); }, (KFR_DFT_PLAN_F64*)nullptr);
+    }
+    void kfr_dft_dump_f32(KFR_DFT_PLAN_F32* plan)
+    {
+        try_void([&] { print_dump(dft_cast<float>(plan)->desc, dft_cast<float>(plan)->g.schedule()); });
+    }
+    void kfr_dft_dump_f64(KFR_DFT_PLAN_F64* plan)
+    {
+        try_void([&] { print_dump(dft_cast<double>(plan)->desc, dft_cast<double>(plan)->g.schedule()); });
+    }
+    size_t kfr_dft_get_size_f32(KFR_DFT_PLAN_F32* plan)
+    {
+        return try_fn([&] { return dft_cast<float>(plan)->g.size(); }, (size_t)0);
+    }
+    size_t kfr_dft_get_size_f64(KFR_DFT_PLAN_F64* plan)
+    {
+        return try_fn([&] { return dft_cast<double>(plan)->g.size(); }, (size_t)0);
+    }
+    size_t kfr_dft_get_temp_size_f32(KFR_DFT_PLAN_F32* plan)
+    {
+        return try_fn([&] { return dft_cast<float>(plan)->desc.temp_size; }, (size_t)0);
+    }
+    size_t kfr_dft_get_temp_size_f64(KFR_DFT_PLAN_F64* plan)
+    {
+        return try_fn([&] { return dft_cast<double>(plan)->desc.temp_size; }, (size_t)0);
+    }
+    void kfr_dft_execute_f32(KFR_DFT_PLAN_F32* plan, kfr_c32* out, const kfr_c32* in, uint8_t*)
+    {
+        try_void([&] { dft_exec<float>(plan, out, in, false); });
+    }
+    void kfr_dft_execute_f64(KFR_DFT_PLAN_F64* plan, kfr_c64* out, const kfr_c64* in, uint8_t*)
+    {
+        try_void([&] { dft_exec<double>(plan, out, in, false); });
+    }
+    void kfr_dft_execute_inverse_f32(KFR_DFT_PLAN_F32* plan, kfr_c32* out, const kfr_c32* in, uint8_t*)
+    {
+        try_void([&] { dft_exec<float>(plan, out, in, true); });
+    }
+    void kfr_dft_execute_inverse_f64(KFR_DFT_PLAN_F64* plan, kfr_c64* out, const kfr_c64* in, uint8_t*)
+    {
+        try_void([&] { dft_exec<double>(plan, out, in, true); });
+    }
+    void kfr_dft_delete_plan_f32(KFR_DFT_PLAN_F32* plan)
+    {
+        try_void([&] { delete reinterpret_cast<dft_obj<float>*>(plan); });
+    }
+    void kfr_dft_delete_plan_f64(KFR_DFT_PLAN_F64* plan)
+    {
+        try_void([&] { delete reinterpret_cast<dft_obj<double>*>(plan); });
+    }
+
+    // ---- real DFT
+    KFR_DFT_REAL_PLAN_F32* kfr_dft_real_create_plan_f32(size_t size, KFR_DFT_PACK_FORMAT fmt)
+    {
+        return try_fn([&] { return reinterpret_cast<KFR_DFT_REAL_PLAN_F32*>(new real_obj<float>(size, (int)fmt)); },
+                      (KFR_DFT_REAL_PLAN_F32*)nullptr);
+    }
+    KFR_DFT_REAL_PLAN_F64* kfr_dft_real_create_plan_f64(size_t size, KFR_DFT_PACK_FORMAT fmt)
+    {
+        return try_fn([&] { return reinterpret_cast<KFR_DFT_REAL_PLAN_F64*>(new real_obj<double>(size, (int)fmt)); },
+                      (KFR_DFT_REAL_PLAN_F64*)nullptr);
+    }
+    KFR_DFT_REAL_PLAN_F32* kfr_dft_real_create_2d_plan_f32(size_t, size_t, kfr_bool)
+    {
+        return try_fn([&]() -> KFR_DFT_REAL_PLAN_F32* { not_provided("multidimensional real DFT"); },
+                      (KFR_DFT_REAL_PLAN_F32*)nullptr);
+    }
+    KFR_DFT_REAL_PLAN_F64* kfr_dft_real_create_2d_plan_f64(size_t, size_t, kfr_bool)
+    {
+        return try_fn([&]() -> KFR_DFT_REAL_PLAN_F64* { not_provided("multidimensional real DFT"); },
+                      (KFR_DFT_REAL_PLAN_F64*)nullptr);
+    }
+    KFR_DFT_REAL_PLAN_F32* kfr_dft_real_create_3d_plan_f32(size_t, size_t, size_t, kfr_bool)
+    {
+        return try_fn([&]() -> KFR_DFT_REAL_PLAN_F32* { not_provided("multidimensional real DFT"); },
+                      (KFR_DFT_REAL_PLAN_F32*)nullptr);
+    }
+    KFR_DFT_REAL_PLAN_F64* kfr_dft_real_create_3d_plan_f64(size_t, size_t, size_t, kfr_bool)
+    {
+        return try_fn([&]() -> KFR_DFT_REAL_PLAN_F64* { not_provided("multidimensional real DFT"); },
+                      (KFR_DFT_REAL_PLAN_F64*)nullptr);
+    }
+    KFR_DFT_REAL_PLAN_F32* kfr_dft_real_create_md_plan_f32(size_t, const unsigned*, kfr_bool)
+    {
+        return try_fn([&]() -> KFR_DFT_REAL_PLAN_F32* { not_provided("multidimensional real DFT"); },
+                      (KFR_DFT_REAL_PLAN_F32*)nullptr);
+    }
+    KFR_DFT_REAL_PLAN_F64* kfr_dft_real_create_md_plan_f64(size_t, const unsigned*, kfr_bool)
+    {
+        return try_fn([&]() -> KFR_DFT_REAL_PLAN_F64* { not_provided("multidimensional real DFT"); },
+                      (KFR_DFT_REAL_PLAN_F64*)nullptr);
+    }
+    void kfr_dft_real_dump_f32(KFR_DFT_REAL_PLAN_F32* plan)
+    {
+        try_void([&] { print_dump(real_cast<float>(plan)->desc, real_cast<float>(plan)->g.schedule()); });
+    }
+    void kfr_dft_real_dump_f64(KFR_DFT_REAL_PLAN_F64* plan)
+    {
+        try_void([&] { print_dump(real_cast<double>(plan)->desc, real_cast<double>(plan)->g.schedule()); });
+    }
+    size_t kfr_dft_real_get_size_f32(KFR_DFT_REAL_PLAN_F32* plan)
+    {
+        return try_fn([&] { return real_cast<float>(plan)->g.size(); }, (size_t)0);
+    }
+    size_t kfr_dft_real_get_size_f64(KFR_DFT_REAL_PLAN_F64* plan)
+    {
+        return try_fn([&] { return real_cast<double>(plan)->g.size(); }, (size_t)0);
+    }
+    size_t kfr_dft_real_get_temp_size_f32(KFR_DFT_REAL_PLAN_F32* plan)
+    {
+        return try_fn([&] { return real_cast<float>(plan)->desc.temp_size; }, (size_t)0);
+    }
+    size_t kfr_dft_real_get_temp_size_f64(KFR_DFT_REAL_PLAN_F64* plan)
+    {
+        return try_fn([&] { return real_cast<double>(plan)->desc.temp_size; }, (size_t)0);
+    }
+    void kfr_dft_real_execute_f32(KFR_DFT_REAL_PLAN_F32* plan, kfr_c32* out, const kfr_f32* in, uint8_t*)
+    {
+        try_void([&] { real_exec<float>(plan, out, in, false); });
+    }
+    void kfr_dft_real_execute_f64(KFR_DFT_REAL_PLAN_F64* plan, kfr_c64* out, const kfr_f64* in, uint8_t*)
+    {
+        try_void([&] { real_exec<double>(plan, out, in, false); });
+    }
+    void kfr_dft_real_execute_inverse_f32(KFR_DFT_REAL_PLAN_F32* plan, kfr_f32* out, const kfr_c32* in, uint8_t*)
+    {
+        try_void([&] { real_exec<float>(plan, out, in, true); });
+    }
+    void kfr_dft_real_execute_inverse_f64(KFR_DFT_REAL_PLAN_F64* plan, kfr_f64* out, const kfr_c64* in, uint8_t*)
+    {
+        try_void([&] { real_exec<double>(plan, out, in, true); });
+    }
+    void kfr_dft_real_delete_plan_f32(KFR_DFT_REAL_PLAN_F32* plan)
+    {
+        try_void([&] { delete reinterpret_cast<real_obj<float>*>(plan); });
+    }
+    void kfr_dft_real_delete_plan_f64(KFR_DFT_REAL_PLAN_F64* plan)
+    {
+        try_void([&] { delete reinterpret_cast<real_obj<double>*>(plan); });
+    }
+
+    // ---- DCT
+    KFR_DCT_PLAN_F32* kfr_dct_create_plan_f32(size_t size)
+    {
+        return try_fn([&] { return reinterpret_cast<KFR_DCT_PLAN_F32*>(new dct_obj<float>(size)); },
+                      (KFR_DCT_PLAN_F32*)nullptr);
+    }
+    KFR_DCT_PLAN_F64* kfr_dct_create_plan_f64(size_t size)
+    {
+        return try_fn([&] { return reinterpret_cast<KFR_DCT_PLAN_F64*>(new dct_obj<double>(size)); },
+                      (KFR_DCT_PLAN_F64*)nullptr);
+    }
+    void kfr_dct_dump_f32(KFR_DCT_PLAN_F32* plan)
+    {
+        try_void([&] { print_dump(dct_cast<float>(plan)->desc, dct_cast<float>(plan)->g.schedule()); });
+    }
+    void kfr_dct_dump_f64(KFR_DCT_PLAN_F64* plan)
+    {
+        try_void([&] { print_dump(dct_cast<double>(plan)->desc, dct_cast<double>(plan)->g.schedule()); });
+    }
+    size_t kfr_dct_get_size_f32(KFR_DCT_PLAN_F32* plan)
+    {
+        return try_fn([&] { return dct_cast<float>(plan)->g.size(); }, (size_t)0);
+    }
+    size_t kfr_dct_get_size_f64(KFR_DCT_PLAN_F64* plan)
+    {
+        return try_fn([&] { return dct_cast<double>(plan)->g.size(); }, (size_t)0);
+    }
+    size_t kfr_dct_get_temp_size_f32(KFR_DCT_PLAN_F32* plan)
+    {
+        return try_fn([&] { return dct_cast<float>(plan)->desc.temp_size; }, (size_t)0);
+    }
+    size_t kfr_dct_get_temp_size_f64(KFR_DCT_PLAN_F64* plan)
+    {
+        return try_fn([&] { return dct_cast<double>(plan)->desc.temp_size; }, (size_t)0);
+    }
+    void kfr_dct_execute_f32(KFR_DCT_PLAN_F32* plan, kfr_f32* out, const kfr_f32* in, uint8_t*)
+    {
+        try_void([&] { dct_exec<float>(plan, out, in, false); });
+    }
+    void kfr_dct_execute_f64(KFR_DCT_PLAN_F64* plan, kfr_f64* out, const kfr_f64* in, uint8_t*)
+    {
+        try_void([&] { dct_exec<double>(plan, out, in, false); });
+    }
+    void kfr_dct_execute_inverse_f32(KFR_DCT_PLAN_F32* plan, kfr_f32* out, const kfr_f32* in, uint8_t*)
+    {
+        try_void([&] { dct_exec<float>(plan, out, in, true); });
+    }
+    void kfr_dct_execute_inverse_f64(KFR_DCT_PLAN_F64* plan, kfr_f64* out, const kfr_f64* in, uint8_t*)
+    {
+        try_void([&] { dct_exec<double>(plan, out, in, true); });
+    }
+    void kfr_dct_delete_plan_f32(KFR_DCT_PLAN_F32* plan)
+    {
+        try_void([&] { delete reinterpret_cast<dct_obj<float>*>(plan); });
+    }
+    void kfr_dct_delete_plan_f64(KFR_DCT_PLAN_F64* plan)
+    {
+        try_void([&] { delete reinterpret_cast<dct_obj<double>*>(plan); });
+    }
+
+    // ---- filters
+    KFR_FILTER_F32* kfr_filter_create_fir_plan_f32(const kfr_f32*, size_t)
+    {
+        return try_fn([&]() -> KFR_FILTER_F32* { not_provided("FIR filter"); }, (KFR_FILTER_F32*)nullptr);
+    }
+    KFR_FILTER_F64* kfr_filter_create_fir_plan_f64(const kfr_f64*, size_t)
+    {
+        return try_fn([&]() -> KFR_FILTER_F64* { not_provided("FIR filter"); }, (KFR_FILTER_F64*)nullptr);
+    }
+    KFR_FILTER_F32* kfr_filter_create_iir_plan_f32(const kfr_f32*, size_t)
+    {
+        return try_fn([&]() -> KFR_FILTER_F32* { not_provided("IIR filter"); }, (KFR_FILTER_F32*)nullptr);
+    }
+    KFR_FILTER_F64* kfr_filter_create_iir_plan_f64(const kfr_f64*, size_t)
+    {
+        return try_fn([&]() -> KFR_FILTER_F64* { not_provided("IIR filter"); }, (KFR_FILTER_F64*)nullptr);
+    }
+    KFR_FILTER_F32* kfr_filter_create_convolution_plan_f32(const kfr_f32* taps, size_t size, size_t block_size)
+    {
+        return try_fn(
+            [&] {
+                return reinterpret_cast<KFR_FILTER_F32*>(
+                    new filter_obj<float>(taps, size, block_size ? block_size : 1024));
+            },
+            (KFR_FILTER_F32*)nullptr);
+    }
+    KFR_FILTER_F64* kfr_filter_create_convolution_plan_f64(const kfr_f64* taps, size_t size, size_t block_size)
+    {
+        return try_fn(
+            [&] {
+                return reinterpret_cast<KFR_FILTER_F64*>(
+                    new filter_obj<double>(taps, size, block_size ? block_size : 1024));
+            },
+            (KFR_FILTER_F64*)nullptr);
+    }
+    void kfr_filter_process_f32(KFR_FILTER_F32* plan, kfr_f32* output, const kfr_f32* input, size_t size)
+    {
+        try_void([&] { filter_process<float>(plan, output, input, size); });
+    }
+    void kfr_filter_process_f64(KFR_FILTER_F64* plan, kfr_f64* output, const kfr_f64* input, size_t size)
+    {
+        try_void([&] { filter_process<double>(plan, output, input, size); });
+    }
+    void kfr_filter_reset_f32(KFR_FILTER_F32* plan)
+    {
+        try_void(
+            [&]
+            {
+                check_plan(plan);
+                reinterpret_cast<filter_obj<float>*>(plan)->g.reset();
+            });
+    }
+    void kfr_filter_reset_f64(KFR_FILTER_F64* plan)
+    {
+        try_void(
+            [&]
+            {
+                check_plan(plan);
+                reinterpret_cast<filter_obj<double>*>(plan)->g.reset();
+            });
+    }
+    void kfr_filter_delete_plan_f32(KFR_FILTER_F32* plan)
+    {
+        try_void([&] { delete reinterpret_cast<filter_obj<float>*>(plan); });
+    }
+    void kfr_filter_delete_plan_f64(KFR_FILTER_F64* plan)
+    {
+        try_void([&] { delete reinterpret_cast<filter_obj<double>*>(plan); });
+    }
+
+    // ---- extensions
+    int kfr_amd_dft_execute_batch_f32(KFR_DFT_PLAN_F32* plan, kfr_c32* out, const kfr_c32* in, size_t batch,
+                                      int inverse, void* stream)
+    {
+        return try_fn(
+            [&]
+            {
+                dft_cast<float>(plan)->g.execute(out, in, (long long)batch, inverse != 0, as_stream(stream));
+                return 0;
+            },
+            -1);
+    }
+    int kfr_amd_dft_execute_batch_f64(KFR_DFT_PLAN_F64* plan, kfr_c64* out, const kfr_c64* in, size_t batch,
+                                      int inverse, void* stream)
+    {
+        return try_fn(
+            [&]
+            {
+                dft_cast<double>(plan)->g.execute(out, in, (long long)batch, inverse != 0, as_stream(stream));
+                return 0;
+            },
+            -1);
+    }
+    int kfr_amd_dft_real_execute_batch_f32(KFR_DFT_REAL_PLAN_F32* plan, void* out, const void* in, size_t batch,
+                                           int inverse, void* stream)
+    {
+        return try_fn(
+            [&]
+            {
+                auto* o = real_cast<float>(plan);
+                if (inverse)
+                    o->g.inverse(out, in, (long long)batch, as_stream(stream));
+                else
+                    o->g.forward(out, in, (long long)batch, as_stream(stream));
+                return 0;
+            },
+            -1);
+    }
+    int kfr_amd_dft_real_execute_batch_f64(KFR_DFT_REAL_PLAN_F64* plan, void* out, const void* in, size_t batch,
+                                           int inverse, void* stream)
+    {
+        return try_fn(
+            [&]
+            {
+                auto* o = real_cast<double>(plan);
+                if (inverse)
+                    o->g.inverse(out, in, (long long)batch, as_stream(stream));
+                else
+                    o->g.forward(out, in, (long long)batch, as_stream(stream));
+                return 0;
+            },
+            -1);
+    }
+    int kfr_amd_dct_execute_batch_f32(KFR_DCT_PLAN_F32* plan, kfr_f32* out, const kfr_f32* in, size_t batch,
+                                      int inverse, void* stream)
+    {
+        return try_fn(
+            [&]
+            {
+                dct_cast<float>(plan)->g.execute(out, in, (long long)batch, inverse != 0, as_stream(stream));
+                return 0;
+            },
+            -1);
+    }
+    int kfr_amd_dct_execute_batch_f64(KFR_DCT_PLAN_F64* plan, kfr_f64* out, const kfr_f64* in, size_t batch,
+                                      int inverse, void* stream)
+    {
+        return try_fn(
+            [&]
+            {
+                dct_cast<double>(plan)->g.execute(out, in, (long long)batch, inverse != 0, as_stream(stream));
+                return 0;
+            },
+            -1);
+    }
+    size_t kfr_amd_dft_real_get_complex_size_f32(KFR_DFT_REAL_PLAN_F32* plan)
+    {
+        return try_fn([&] { return real_cast<float>(plan)->g.complex_size(); }, (size_t)0);
+    }
+    size_t kfr_amd_dft_real_get_complex_size_f64(KFR_DFT_REAL_PLAN_F64* plan)
+    {
+        return try_fn([&] { return real_cast<double>(plan)->g.complex_size(); }, (size_t)0);
+    }
+
+    long kfr_amd_convolve(int dtype, int mode, const void* a, size_t na, const void* b, size_t nb, void* out)
+    {
+        return try_fn(
+            [&]() -> long
+            {
+                if (mode < 0 || mode > 2)
+                    throw std::invalid_argument("mode must be 0 (convolve), 1 (correlate) or 2 (autocorrelate)");
+                switch (dtype)
+                {
+                case 0:
+                    return convolve_impl<float>(mode, a, na, b, nb, out, false);
+                case 1:
+                    return convolve_impl<double>(mode, a, na, b, nb, out, false);
+                case 2:
+                    return convolve_impl<float>(mode, a, na, b, nb, out, true);
+                case 3:
+                    return convolve_impl<double>(mode, a, na, b, nb, out, true);
+                default:
+                    throw std::invalid_argument("dtype must be 0..3");
+                }
+            },
+            -1L);
+    }
+
+    KFR_AMD_CONV* kfr_amd_conv_create(int dtype, size_t nx, size_t nh)
+    {
+        return try_fn(
+            [&]
+            {
+                auto* c  = new conv_obj();
+                c->dtype = (int)to_dtype(dtype);
+                try
+                {
+                    if (dtype == 0)
+                        c->f.reset(new gpu_conv<float>(nx, nh));
+                    else
+                        c->d.reset(new gpu_conv<double>(nx, nh));
+                }
+                catch (...)
+                {
+                    delete c;
+                    throw;
+                }
+                return reinterpret_cast<KFR_AMD_CONV*>(c);
+            },
+            (KFR_AMD_CONV*)nullptr);
+    }
+    size_t kfr_amd_conv_fft_size(KFR_AMD_CONV* conv)
+    {
+        return try_fn(
+            [&]
+            {
+                check_plan(conv);
+                auto* c = reinterpret_cast<conv_obj*>(conv);
+                return c->f ? c->f->fft_size() : c->d->fft_size();
+            },
+            (size_t)0);
+    }
+    int kfr_amd_conv_forward(KFR_AMD_CONV* conv, void* y, const void* x, const void* h, size_t h_dist, size_t batch,
+                             int correlate, void* stream)
+    {
+        return try_fn(
+            [&]
+            {
+                check_plan(conv);
+                auto* c = reinterpret_cast<conv_obj*>(conv);
+                std::lock_guard<std::mutex> lock(c->mu);
+                if (c->f)
+                    c->f->forward(y, x, h, (long long)h_dist, (long long)batch, correlate != 0, as_stream(stream));
+                else
+                    c->d->forward(y, x, h, (long long)h_dist, (long long)batch, correlate != 0, as_stream(stream));
+                return 0;
+            },
+            -1);
+    }
+    int kfr_amd_conv_backward(KFR_AMD_CONV* conv, void* dx, void* dh, const void* g, const void* x, const void* h,
+                              size_t h_dist, size_t batch, void* stream)
+    {
+        return try_fn(
+            [&]
+            {
+                check_plan(conv);
+                auto* c = reinterpret_cast<conv_obj*>(conv);
+                std::lock_guard<std::mutex> lock(c->mu);
+                if (c->f)
+                    c->f->backward(dx, dh, g, x, h, (long long)h_dist, (long long)batch, as_stream(stream));
+                else
+                    c->d->backward(dx, dh, g, x, h, (long long)h_dist, (long long)batch, as_stream(stream));
+                return 0;
+            },
+            -1);
+    }
+    void kfr_amd_conv_delete(KFR_AMD_CONV* conv)
+    {
+        try_void([&] { delete reinterpret_cast<conv_obj*>(conv); });
+    }
+
+    int kfr_amd_describe(int kind, int dtype, size_t size, int arg, char* buf, size_t buflen)
+    {
+        return try_fn(
+            [&]
+            {
+                const dtype_t t = to_dtype(dtype);
+                plan_desc d;
+                switch (kind)
+                {
+                case 0:
+                    d = describe_dft(t, size, false);
+                    break;
+                case 1:
+                    d = describe_dft_real(t, size, arg);
+                    break;
+                case 2:
+                    d = describe_dct(t, size);
+                    break;
+                case 3:
+                    d = describe_dft(t, size, true);
+                    break;
+                default:
+                    throw std::invalid_argument("kind must be 0..3");
+                }
+                char line[256];
+                std::snprintf(line, sizeof(line), "size=%zu temp_size=%zu data_size=%zu stages=%zu,%zu arblen=%d\n",
+                              d.size, d.temp_size, d.data_size, d.order[0].size(), d.order[1].size(), (int)d.arblen);
+                std::string s = line;
+                s += dump_text(d, "gfx950");
+                for (int dir = 0; dir < 2; ++dir)
+                {
+                    s += dir == 0 ? "fwd:" : "inv:";
+                    for (int i : d.order[dir])
+                        s += " " + std::to_string(i);
+                    s += "\n";
+                }
+                return copy_text(s, buf, buflen);
+            },
+            -1);
+    }
+
+    int kfr_amd_dft_permutation(int dtype, size_t size, int64_t* perm)
+    {
+        return try_fn(
+            [&]
+            {
+                const auto p = reorder_permutation(describe_dft(to_dtype(dtype), size, false));
+                if (p.empty())
+                    return 0;
+                std::copy(p.begin(), p.end(), perm);
+                return 1;
+            },
+            -1);
+    }
+
+    size_t kfr_amd_dft_progressive_total_steps(int dtype, size_t size)
+    {
+        return try_fn([&] { return describe_dft(to_dtype(dtype), size, false).order[0].size(); }, (size_t)0);
+    }
+
+    int kfr_amd_dft_schedule(int dtype, size_t size, char* buf, size_t buflen)
+    {
+        return try_fn(
+            [&]
+            {
+                if (to_dtype(dtype) == dtype_t::f32)
+                    return copy_text(gpu_dft<float>(size).schedule().text(), buf, buflen);
+                return copy_text(gpu_dft<double>(size).schedule().text(), buf, buflen);
+            },
+            -1);
+    }
+
+    void kfr_amd_inject_alloc_failure(int n) { set_alloc_failure_countdown(n); }
+
+    int kfr_amd_synchronize(void)
+    {
+        return try_fn(
+            []
+            {
+                hip_check(hipStreamSynchronize(library_stream()), "synchronize");
+                return 0;
+            },
+            -1);
+    }
+
+} // extern "C"

@@ -1,0 +1,749 @@
+// Core GPU transform machinery shared by all plans: FP64 table generation, the LDS Stockham kernels
+// with fused element operations, the four-step two-pass transform for lengths beyond one workgroup,
+// and the schedule (radix choice and launch geometry) for every length.
+//
+// Element operations connect a transform to its data: the load phase calls ld.load(b, e) for element
+// e of batch item b, and the store phase calls st.store(b, e, v). Bluestein chirps, DCT mirroring,
+// real-spectrum repacking and pointwise spectrum products are element operations, so none of them
+// costs an extra trip through HBM. They are selected at run time (a kernel-uniform switch), which
+// keeps the number of compiled kernels small.
+#pragma once
+
+#include "fft_device.hpp"
+#include "fft_engine.hpp"
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace kfr_amd
+{
+
+inline unsigned nblocks(long long count, int bs) { return (unsigned)((count + bs - 1) / bs); }
+
+// kernels that use more than 64 KiB of dynamic LDS (up to the CU's 160 KiB) must say so first
+template <typename K>
+inline void allow_lds(K kernel, size_t bytes)
+{
+    if (bytes > 65536)
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes),
+                  "dynamic LDS attribute");
+}
+
+// ------------------------------------------------------------------------------------------------
+// element operations
+
+enum ld_mode : int
+{
+    ld_plain    = 0, // p[b*pdist + e]
+    ld_real_pad = 1, // real pairs p[2e], p[2e+1] of a zero-padded (optionally reversed) real signal
+    ld_dct2     = 2, // (x[mirror(e)], 0)
+    ld_dct3     = 3, // x[e] * W_{4n}^e (x[0] halved)
+    ld_c2r      = 4, // from_fmt: packed half-length sequence from a CCs/Perm half spectrum
+    ld_cpad     = 5, // zero-padded (optionally reversed) complex signal
+    ld_mul      = 6  // p[b*pdist + e] * q[e] (flag: conj(q[e]))
+};
+
+enum st_mode : int
+{
+    st_plain      = 0, // p[b*pdist + e] = v
+    st_real_trunc = 1, // real pairs, truncated to n reals, scaled
+    st_dct2       = 2, // y[e] = Re(v * W_{4n}^e)
+    st_dct3       = 3, // y[mirror(e)] = Re(v)
+    st_ctrunc     = 4  // complex, truncated to n, scaled; flag = keep imaginary part
+};
+
+__device__ __forceinline__ long long dct_mirror(long long e, long long n)
+{
+    return e < (n + 1) / 2 ? 2 * e : 2 * (n - 1 - e) + 1;
+}
+
+template <typename T>
+struct xload
+{
+    int mode      = ld_plain;
+    int flag      = 0; // real_pad/cpad: reversed; c2r: pack format (1 CCs, 0 Perm); mul: conjugate q
+    int src       = 0; // c2r spectrum source: 0 p, 1 p*q*scale, 2 p*conj(q)*scale
+    int mconj     = 0; // conjugate the post-multiplier
+    const void* p = nullptr;
+    long long pdist  = 0;
+    const cx<T>* q   = nullptr;
+    long long qdist  = 0;
+    const cx<T>* r   = nullptr; // c2r twiddles
+    long long n      = 0;
+    T scale          = T(1);
+    long long b0     = 0;
+    const cx<T>* mtab = nullptr; // post-multiplier table (Bluestein chirp)
+    long long lim    = LLONG_MAX; // elements at or beyond lim are zero
+
+    __device__ cx<T> spec(long long b, long long k) const
+    {
+        const cx<T> a = static_cast<const cx<T>*>(p)[b * pdist + k];
+        if (src == 0)
+            return a;
+        const cx<T> c = q[b * qdist + k];
+        return cscale(src == 1 ? cmul(a, c) : cmulc(a, c), scale);
+    }
+
+    __device__ cx<T> base(long long b, long long e) const
+    {
+        switch (mode)
+        {
+        case ld_plain:
+        default:
+            return static_cast<const cx<T>*>(p)[b * pdist + e];
+        case ld_real_pad:
+        {
+            const T* x         = static_cast<const T*>(p) + b * pdist;
+            const long long i0 = 2 * e, i1 = 2 * e + 1;
+            T a = T(0), c = T(0);
+            if (i0 < n)
+                a = flag ? x[n - 1 - i0] : x[i0];
+            if (i1 < n)
+                c = flag ? x[n - 1 - i1] : x[i1];
+            return { a, c };
+        }
+        case ld_dct2:
+            return { static_cast<const T*>(p)[b * n + dct_mirror(e, n)], T(0) };
+        case ld_dct3:
+        {
+            const T v = static_cast<const T*>(p)[b * n + e] * (e == 0 ? T(0.5) : T(1));
+            return { v * q[e].re, v * q[e].im };
+        }
+        case ld_c2r:
+        {
+            // from_fmt (src/dft/fft-impl.hpp:1853-1896); n = half length
+            const long long cs = n;
+            if (e == 0)
+            {
+                const cx<T> x0 = spec(b, 0);
+                if (flag == 1)
+                {
+                    const cx<T> xn = spec(b, cs);
+                    return { x0.re + xn.re, x0.re - xn.re };
+                }
+                return { x0.re + x0.im, x0.re - x0.im };
+            }
+            if (2 * e == cs)
+            {
+                const cx<T> xm = spec(b, e);
+                return { T(2) * xm.re, T(-2) * xm.im };
+            }
+            const bool lo     = 2 * e < cs;
+            const long long k = lo ? e : cs - e;
+            const cx<T> xk    = spec(b, k);
+            const cx<T> xnk   = spec(b, cs - k);
+            const cx<T> f1    = cadd(xk, cconj(xnk));
+            const cx<T> f2    = csub(xk, cconj(xnk));
+            const cx<T> t     = cmulc(f2, r[k]);
+            return lo ? cadd(f1, t) : cconj(csub(f1, t));
+        }
+        case ld_cpad:
+        {
+            if (e >= n)
+                return { T(0), T(0) };
+            return static_cast<const cx<T>*>(p)[b * pdist + (flag ? n - 1 - e : e)];
+        }
+        case ld_mul:
+        {
+            const cx<T> a = static_cast<const cx<T>*>(p)[b * pdist + e];
+            return flag ? cmulc(a, q[e]) : cmul(a, q[e]);
+        }
+        }
+    }
+
+    __device__ __forceinline__ cx<T> load(long long b, long long e) const
+    {
+        if (e >= lim)
+            return { T(0), T(0) };
+        cx<T> v = base(b + b0, e);
+        if (mtab)
+            v = mconj ? cmulc(v, mtab[e]) : cmul(v, mtab[e]);
+        return v;
+    }
+};
+
+template <typename T>
+struct xstore
+{
+    int mode       = st_plain;
+    int flag       = 0;
+    int mconj      = 0;
+    void* p        = nullptr;
+    long long pdist = 0;
+    const cx<T>* q = nullptr; // dct table
+    long long n    = 0;
+    T scale        = T(1);
+    long long b0   = 0;
+    const cx<T>* mtab = nullptr; // pre-multiplier (Bluestein chirp)
+    long long lim  = LLONG_MAX;  // elements at or beyond lim are dropped
+
+    __device__ void base(long long b, long long e, cx<T> v) const
+    {
+        switch (mode)
+        {
+        case st_plain:
+        default:
+            static_cast<cx<T>*>(p)[b * pdist + e] = v;
+            break;
+        case st_real_trunc:
+        {
+            T* y               = static_cast<T*>(p) + b * pdist;
+            const long long i0 = 2 * e;
+            if (i0 < n)
+                y[i0] = v.re * scale;
+            if (i0 + 1 < n)
+                y[i0 + 1] = v.im * scale;
+            break;
+        }
+        case st_dct2:
+        {
+            const cx<T> w = q[e];
+            static_cast<T*>(p)[b * n + e] = v.re * w.re - v.im * w.im;
+            break;
+        }
+        case st_dct3:
+            static_cast<T*>(p)[b * n + dct_mirror(e, n)] = v.re;
+            break;
+        case st_ctrunc:
+            if (e < n)
+                static_cast<cx<T>*>(p)[b * pdist + e] = { v.re * scale, flag ? v.im * scale : T(0) };
+            break;
+        }
+    }
+
+    __device__ __forceinline__ void store(long long b, long long e, cx<T> v) const
+    {
+        if (e >= lim)
+            return;
+        if (mtab)
+            v = mconj ? cmulc(v, mtab[e]) : cmul(v, mtab[e]);
+        base(b + b0, e, v);
+    }
+};
+
+template <typename T>
+inline xload<T> load_plain(const void* p, long long dist)
+{
+    xload<T> l;
+    l.p     = p;
+    l.pdist = dist;
+    return l;
+}
+template <typename T>
+inline xstore<T> store_plain(void* p, long long dist)
+{
+    xstore<T> s;
+    s.p     = p;
+    s.pdist = dist;
+    return s;
+}
+
+// ------------------------------------------------------------------------------------------------
+// FP64 table generation: out[i] = exp(-2*pi*i*a_i/M), a_i reduced exactly in integers, evaluated
+// with FP64 sincospi and rounded once to T.
+
+enum twiddle_mode : int
+{
+    tw_linear = 0, // a = (i*mult + add) mod M
+    tw_radix  = 1, // radix-pass table: i = k*(R-1) + (r-1), a = r*k; M = Ns*R
+    tw_square = 2  // a = i^2 mod M (Bluestein chirp, M = 2N)
+};
+
+template <typename T>
+__global__ void k_twiddles(cx<T>* out, long long count, long long M, long long mult, long long add, int mode,
+                           int R)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count)
+        return;
+    unsigned long long a;
+    if (mode == tw_radix)
+    {
+        const long long k = i / (R - 1);
+        const long long r = i - k * (R - 1) + 1;
+        a                 = (unsigned long long)((r * k) % M);
+    }
+    else if (mode == tw_square)
+    {
+        const unsigned long long ii = (unsigned long long)i % (unsigned long long)M;
+        a = (unsigned long long)(((unsigned __int128)ii * ii) % (unsigned long long)M);
+    }
+    else
+    {
+        a = (unsigned long long)(((__int128)i * mult + add) % M);
+    }
+    long long as = (long long)a; // symmetric range keeps the FP64 argument small
+    if (2 * as > M)
+        as -= M;
+    const double x = 2.0 * (double)as / (double)M;
+    double s, c;
+    sincospi(x, &s, &c);
+    out[i] = { (T)c, (T)(-s) };
+}
+
+template <typename T>
+void gen_twiddles(cx<T>* out, long long count, long long M, long long mult, long long add, int mode, int R,
+                  hipStream_t s)
+{
+    if (count <= 0)
+        return;
+    hipLaunchKernelGGL(k_twiddles<T>, dim3(nblocks(count, 256)), dim3(256), 0, s, out, count, M, mult, add, mode,
+                       R);
+    hip_check(hipGetLastError(), "twiddle generation launch");
+}
+
+// ------------------------------------------------------------------------------------------------
+// Stockham schedule (host): GPU radix choice, twiddle layout and launch geometry for one length.
+
+struct stockham
+{
+    int n = 0;
+    std::vector<int> radix, kind, ns;
+    std::vector<long long> tw_off;
+    long long tw_count = 0;
+    int tpt            = 0;
+};
+
+inline bool lds_radix(int r) { return r == 2 || r == 3 || r == 4 || r == 5 || r == 7 || r == 8 || r == 16; }
+
+// Factor n into GPU passes. Power-of-2 part: radix 16 first, remainder 8/4/2; then 7, 5, 3; any
+// other prime factor up to 127 becomes a direct DFT pass, placed last.
+inline bool make_stockham(long long nn, stockham& S)
+{
+    S = stockham{};
+    if (nn < 1 || nn > (1 << 28))
+        return false;
+    const int n = (int)nn;
+    S.n         = n;
+    int m       = n;
+    std::vector<int> pw2, odd, gen;
+    int a = 0;
+    while (m % 2 == 0)
+    {
+        m /= 2;
+        ++a;
+    }
+    while (a >= 4)
+    {
+        pw2.push_back(16);
+        a -= 4;
+    }
+    if (a == 3)
+        pw2.push_back(8);
+    else if (a == 2)
+        pw2.push_back(4);
+    else if (a == 1)
+        pw2.push_back(2);
+    for (int p : { 7, 5, 3 })
+        while (m % p == 0)
+        {
+            odd.push_back(p);
+            m /= p;
+        }
+    for (int p = 11; m > 1 && p <= 127; p += 2)
+        while (m % p == 0)
+        {
+            gen.push_back(p);
+            m /= p;
+        }
+    if (m != 1)
+        return false;
+    for (int r : pw2)
+        S.radix.push_back(r);
+    for (int r : odd)
+        S.radix.push_back(r);
+    for (int r : gen)
+        S.radix.push_back(r);
+    if ((int)S.radix.size() > max_passes)
+        return false;
+    long long ns = 1, off = 0;
+    int tpt      = 1;
+    for (int r : S.radix)
+    {
+        const bool g = !lds_radix(r);
+        S.kind.push_back(g ? pass_generic : pass_radix);
+        S.ns.push_back((int)ns);
+        S.tw_off.push_back(off);
+        if (g)
+        {
+            off += ns * r;
+            tpt = std::max(tpt, (n + pmax - 1) / pmax);
+        }
+        else
+        {
+            if (ns > 1)
+                off += ns * (r - 1);
+            const int M  = std::max(1, pmax / r);
+            const int nb = n / r;
+            tpt          = std::max(tpt, (nb + M - 1) / M);
+        }
+        ns *= r;
+    }
+    S.tw_count = off;
+    S.tpt      = tpt;
+    return true;
+}
+
+template <typename T>
+void gen_stockham_twiddles(const stockham& S, cx<T>* tw, hipStream_t s)
+{
+    for (size_t p = 0; p < S.radix.size(); ++p)
+    {
+        const int r  = S.radix[p];
+        const int ns = S.ns[p];
+        if (S.kind[p] == pass_generic)
+            gen_twiddles(tw + S.tw_off[p], (long long)ns * r, (long long)ns * r, 1, 0, tw_linear, r, s);
+        else if (ns > 1)
+            gen_twiddles(tw + S.tw_off[p], (long long)ns * (r - 1), (long long)ns * r, 0, 0, tw_radix, r, s);
+    }
+}
+
+inline lds_geom make_geom(const stockham& S, int tpb, int mapping, int ls, int ld)
+{
+    lds_geom G{};
+    G.n       = S.n;
+    G.np      = (int)S.radix.size();
+    G.tpt     = S.tpt;
+    G.tpb     = tpb;
+    G.mapping = mapping;
+    G.ls      = ls;
+    G.ld      = ld;
+    for (int p = 0; p < G.np; ++p)
+    {
+        G.pass[p].radix  = S.radix[p];
+        G.pass[p].ns     = S.ns[p];
+        G.pass[p].tw     = (int)S.tw_off[p];
+        G.pass[p].kind   = S.kind[p];
+        G.pass[p].ns_div = make_fastdiv((uint32_t)S.ns[p]);
+    }
+    return G;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Kernels
+
+enum kernel_role : int
+{
+    role_direct = 0, // transform g is batch item g; contiguous block load and store
+    role_col    = 1, // four-step pass A: column transforms, twiddled, into the intermediate
+    role_row    = 2  // four-step pass B: row transforms of the intermediate, transposed store
+};
+
+template <typename T>
+struct fourstep_args
+{
+    cx<T>* scratch = nullptr;
+    const cx<T>* ta = nullptr; // W_n^(h*2^lbits)
+    const cx<T>* tb = nullptr; // W_n^l, l < 2^lbits
+    fastdiv n1div, n2div, ndiv;
+    int n1 = 1, n2 = 1, lbits = 0;
+    long long n = 1;
+};
+
+// contiguous load of the block's transforms into LDS (element fastest)
+template <typename T>
+__device__ __forceinline__ void block_load(const lds_geom& G, cx<T>* lds, long long first, long long batch,
+                                           const xload<T>& ld, const fastdiv& ndiv)
+{
+    const int total = G.tpb * G.n;
+    for (int e = threadIdx.x; e < total; e += blockDim.x)
+    {
+        const int tb      = (int)fdiv((uint32_t)e, ndiv);
+        const int i       = e - tb * G.n;
+        const long long g = first + tb;
+        lds[(size_t)tb * G.ld + (size_t)i * G.ls] = g < batch ? ld.load(g, i) : cx<T>{ T(0), T(0) };
+    }
+}
+template <typename T>
+__device__ __forceinline__ void block_store(const lds_geom& G, const cx<T>* lds, long long first, long long batch,
+                                            const xstore<T>& st, const fastdiv& ndiv)
+{
+    const int total = G.tpb * G.n;
+    for (int e = threadIdx.x; e < total; e += blockDim.x)
+    {
+        const int tb      = (int)fdiv((uint32_t)e, ndiv);
+        const int i       = e - tb * G.n;
+        const long long g = first + tb;
+        if (g < batch)
+            st.store(g, i, lds[(size_t)tb * G.ld + (size_t)i * G.ls]);
+    }
+}
+
+template <typename T, bool INV, int ROLE>
+__global__ void __launch_bounds__(1024) k_fft(lds_geom G, const cx<T>* __restrict__ tw, long long batch,
+                                              xload<T> ld, xstore<T> st, fourstep_args<T> fs)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* lds            = reinterpret_cast<cx<T>*>(smem_raw);
+    const long long first = (long long)blockIdx.x * G.tpb;
+    const fft_ctx<T> c    = make_ctx<T>(G, lds, first, batch);
+    if constexpr (ROLE == role_direct)
+    {
+        block_load(G, lds, first, batch, ld, fs.ndiv);
+    }
+    else if constexpr (ROLE == role_col)
+    {
+        const uint32_t bl = fdiv((uint32_t)c.g, fs.n1div);
+        const int col     = (int)((uint32_t)c.g - bl * (uint32_t)fs.n1);
+        for (int i = c.t; i < G.n; i += G.tpt)
+            c.L[i * G.ls] = c.valid ? ld.load(bl, (long long)col + (long long)fs.n1 * i) : cx<T>{ T(0), T(0) };
+    }
+    else
+    {
+        const int total = G.tpb * G.n;
+        for (int e = threadIdx.x; e < total; e += blockDim.x)
+        {
+            const int tb      = (int)fdiv((uint32_t)e, fs.ndiv);
+            const int i       = e - tb * G.n;
+            const long long g = first + tb;
+            lds[(size_t)tb * G.ld + i] = g < batch ? fs.scratch[g * G.n + i] : cx<T>{ T(0), T(0) };
+        }
+    }
+    __syncthreads();
+    fft_passes<T, INV>(G, tw, c);
+    if constexpr (ROLE == role_direct)
+    {
+        block_store(G, lds, first, batch, st, fs.ndiv);
+    }
+    else if constexpr (ROLE == role_col)
+    {
+        if (c.valid)
+        {
+            const uint32_t bl = fdiv((uint32_t)c.g, fs.n1div);
+            const int col     = (int)((uint32_t)c.g - bl * (uint32_t)fs.n1);
+            cx<T>* dst        = fs.scratch + (long long)bl * fs.n + col;
+            const unsigned long long mask = (1ull << fs.lbits) - 1;
+            for (int i = c.t; i < G.n; i += G.tpt)
+            {
+                const unsigned long long m = (unsigned long long)col * (unsigned long long)i; // < n
+                const cx<T> w              = cmul(fs.ta[m >> fs.lbits], fs.tb[m & mask]);
+                dst[(long long)fs.n1 * i]  = ctw<INV>(c.L[i * G.ls], w);
+            }
+        }
+    }
+    else
+    {
+        // transform index fastest: consecutive lanes write consecutive output columns
+        const int total = G.tpb * G.n;
+        for (int e = threadIdx.x; e < total; e += blockDim.x)
+        {
+            const int i       = e / G.tpb;
+            const int tb      = e - i * G.tpb;
+            const long long g = first + tb;
+            if (g < batch)
+            {
+                const uint32_t bl = fdiv((uint32_t)g, fs.n2div);
+                const int k2      = (int)((uint32_t)g - bl * (uint32_t)fs.n2);
+                st.store(bl, (long long)k2 + (long long)fs.n2 * i, lds[(size_t)tb * G.ld + i]);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// A transform computed without Bluestein: one LDS kernel, or four-step in two kernels.
+
+inline size_t lds_budget() { return 128 * 1024; }
+
+template <typename T>
+struct core_fft
+{
+    size_t n      = 0;
+    bool fourstep = false;
+    // single LDS kernel
+    stockham S;
+    int tpb = 1;
+    // four-step: n = n1 * n2; pass A = n2-point columns (C per block), pass B = n1-point rows (R per block)
+    size_t n1 = 0, n2 = 0;
+    stockham SA, SB;
+    int C = 1, R = 1;
+    int lbits       = 0;
+    long long chunk = 1; // batch items per four-step round (intermediate = chunk*n)
+    dev_mem tw;
+    long long twA = 0, twB = 0, twTA = 0, twTB = 0;
+    dev_mem scratch;
+
+    static bool fits_lds(size_t len)
+    {
+        stockham s;
+        if (!make_stockham((long long)len, s))
+            return false;
+        return s.tpt <= 1024 && len * sizeof(cx<T>) <= lds_budget();
+    }
+
+    static int pick_tpb(const stockham& s)
+    {
+        if (s.tpt >= 256)
+            return 1;
+        int t = std::max(1, 256 / s.tpt);
+        // at most 64 KiB of LDS per block so that two blocks share a CU
+        const size_t per = (size_t)s.n * sizeof(cx<T>);
+        t                = (int)std::max<size_t>(1, std::min<size_t>(t, 65536 / per));
+        return t;
+    }
+
+    core_fft(size_t len, hipStream_t s)
+    {
+        n = len;
+        if (fits_lds(n))
+        {
+            make_stockham((long long)n, S);
+            tpb = pick_tpb(S);
+            tw.reset((size_t)std::max<long long>(1, S.tw_count) * sizeof(cx<T>));
+            gen_stockham_twiddles(S, tw.as<cx<T>>(), s);
+            return;
+        }
+        plan_fourstep();
+        fourstep            = true;
+        lbits               = (ilog2_ceil(n) + 1) / 2;
+        const long long nTB = 1ll << lbits;
+        const long long nTA = (long long)(n >> lbits) + 1;
+        twA                 = 0;
+        twB                 = SA.tw_count;
+        twTA                = twB + SB.tw_count;
+        twTB                = twTA + nTA;
+        tw.reset((size_t)(twTB + nTB) * sizeof(cx<T>));
+        cx<T>* t = tw.as<cx<T>>();
+        gen_stockham_twiddles(SA, t + twA, s);
+        gen_stockham_twiddles(SB, t + twB, s);
+        gen_twiddles(t + twTA, nTA, (long long)n, nTB, 0, tw_linear, 0, s);
+        gen_twiddles(t + twTB, nTB, (long long)n, 1, 0, tw_linear, 0, s);
+        size_t cap = (size_t)256 << 20;
+        if (const char* e = std::getenv("KFR_AMD_FOURSTEP_CHUNK_MB"))
+            cap = (size_t)std::max(1, atoi(e)) << 20;
+        chunk = (long long)std::max<size_t>(1, cap / (n * sizeof(cx<T>)));
+    }
+
+    static int ilog2_ceil(size_t v)
+    {
+        int l = 0;
+        while ((size_t(1) << l) < v)
+            ++l;
+        return l;
+    }
+
+    // choose n = n1 * n2 maximising the narrower coalescing width (C columns in pass A, R rows in
+    // pass B), then the balance of the two lengths
+    void plan_fourstep()
+    {
+        size_t force_n2 = 0;
+        if (const char* e = std::getenv("KFR_AMD_FOURSTEP_N2"))
+            force_n2 = (size_t)atoll(e);
+        size_t best_n2 = 0;
+        int best_w = -1, best_C = 1, best_R = 1;
+        double best_bal = 1e300;
+        for (size_t d = 2; d * 2 <= n && d <= ((size_t)1 << 16); ++d)
+        {
+            if (n % d != 0 || (force_n2 && d != force_n2))
+                continue;
+            const size_t a = n / d;
+            if (!fits_lds(d) || !fits_lds(a))
+                continue;
+            stockham sa, sb;
+            make_stockham((long long)d, sa);
+            make_stockham((long long)a, sb);
+            int C = 16;
+            while (C > 1 && ((size_t)C * d * sizeof(cx<T>) > lds_budget() || C * sa.tpt > 1024 || a % C != 0))
+                C /= 2;
+            int R = 16;
+            while (R > 1 &&
+                   ((size_t)R * (a + 1) * sizeof(cx<T>) > lds_budget() || R * sb.tpt > 1024 || d % R != 0))
+                R /= 2;
+            const int w      = std::min(C, R);
+            const double bal = std::fabs(std::log((double)a) - std::log((double)d));
+            if (w > best_w || (w == best_w && bal < best_bal))
+            {
+                best_w   = w;
+                best_bal = bal;
+                best_n2  = d;
+                best_C   = C;
+                best_R   = R;
+            }
+        }
+        if (!best_n2)
+            throw std::runtime_error("transform length " + std::to_string(n) +
+                                     " has no two-pass split on this device");
+        n2 = best_n2;
+        n1 = n / n2;
+        C  = best_C;
+        R  = best_R;
+        if (const char* e = std::getenv("KFR_AMD_FOURSTEP_C"))
+            C = std::max(1, std::min(C, atoi(e)));
+        if (const char* e = std::getenv("KFR_AMD_FOURSTEP_R"))
+            R = std::max(1, std::min(R, atoi(e)));
+        make_stockham((long long)n2, SA);
+        make_stockham((long long)n1, SB);
+    }
+
+    template <bool INV>
+    void run(const xload<T>& ld, const xstore<T>& st, long long batch, hipStream_t s)
+    {
+        if (batch <= 0)
+            return;
+        const cx<T>* t = tw.as<cx<T>>();
+        if (!fourstep)
+        {
+            lds_geom G      = make_geom(S, tpb, 0, 1, (int)n);
+            const size_t sh = (size_t)tpb * n * sizeof(cx<T>);
+            fourstep_args<T> fs;
+            fs.ndiv = make_fastdiv((uint32_t)n);
+            allow_lds(k_fft<T, INV, role_direct>, sh);
+            hipLaunchKernelGGL((k_fft<T, INV, role_direct>), dim3(nblocks(batch, tpb)), dim3(S.tpt * tpb), sh, s, G, t,
+                               batch, ld, st, fs);
+            hip_check(hipGetLastError(), "fft launch");
+            return;
+        }
+        scratch.ensure((size_t)std::min<long long>(chunk, batch) * n * sizeof(cx<T>));
+        lds_geom GA      = make_geom(SA, C, 1, C, 1);
+        lds_geom GB      = make_geom(SB, R, 0, 1, (int)n1 + 1);
+        const size_t shA = (size_t)C * n2 * sizeof(cx<T>);
+        const size_t shB = (size_t)R * (n1 + 1) * sizeof(cx<T>);
+        fourstep_args<T> fs;
+        fs.scratch = scratch.as<cx<T>>();
+        fs.ta      = t + twTA;
+        fs.tb      = t + twTB;
+        fs.n1div   = make_fastdiv((uint32_t)n1);
+        fs.n2div   = make_fastdiv((uint32_t)n2);
+        fs.n1      = (int)n1;
+        fs.n2      = (int)n2;
+        fs.lbits   = lbits;
+        fs.n       = (long long)n;
+        allow_lds(k_fft<T, INV, role_col>, shA);
+        allow_lds(k_fft<T, INV, role_row>, shB);
+        for (long long b0 = 0; b0 < batch; b0 += chunk)
+        {
+            const long long nb = std::min(chunk, batch - b0);
+            xload<T> la        = ld;
+            la.b0 += b0;
+            xstore<T> sb = st;
+            sb.b0 += b0;
+            const long long ga = nb * (long long)n1;
+            fs.ndiv            = make_fastdiv((uint32_t)n2);
+            hipLaunchKernelGGL((k_fft<T, INV, role_col>), dim3(nblocks(ga, C)), dim3(SA.tpt * C), shA, s, GA, t + twA,
+                               ga, la, st, fs);
+            hip_check(hipGetLastError(), "four-step column pass launch");
+            const long long gb = nb * (long long)n2;
+            fs.ndiv            = make_fastdiv((uint32_t)n1);
+            hipLaunchKernelGGL((k_fft<T, INV, role_row>), dim3(nblocks(gb, R)), dim3(SB.tpt * R), shB, s, GB, t + twB,
+                               gb, ld, sb, fs);
+            hip_check(hipGetLastError(), "four-step row pass launch");
+        }
+    }
+
+    std::vector<int> radices() const
+    {
+        if (!fourstep)
+            return S.radix;
+        std::vector<int> r = SA.radix;
+        r.insert(r.end(), SB.radix.begin(), SB.radix.end());
+        return r;
+    }
+};
+
+} // namespace kfr_amd

@@ -1,0 +1,942 @@
+// gfx950 DFT engine: complex, real and DCT plans, Bluestein, FFT convolution and its adjoint.
+#include "fft_core.hpp"
+#include "plan_desc.hpp"
+
+#include <atomic>
+#include <cstdio>
+
+namespace kfr_amd
+{
+
+// ------------------------------------------------------------------------------------------------
+// runtime helpers
+
+static std::atomic<int> g_alloc_countdown{ 0 };
+
+void set_alloc_failure_countdown(int n) { g_alloc_countdown.store(n); }
+
+void hip_check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess)
+        throw hip_error(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+}
+
+dev_mem::~dev_mem()
+{
+    if (ptr)
+        (void)hipFree(ptr);
+}
+
+void dev_mem::reset(size_t b)
+{
+    if (ptr)
+    {
+        hip_check(hipFree(ptr), "hipFree");
+        ptr   = nullptr;
+        bytes = 0;
+    }
+    if (b == 0)
+        return;
+    const int c = g_alloc_countdown.load();
+    if (c > 0 && g_alloc_countdown.fetch_sub(1) == 1)
+        throw hip_error("HIP error in hipMalloc: out of memory (injected allocation failure)");
+    hip_check(hipMalloc(&ptr, b), "hipMalloc");
+    bytes = b;
+}
+
+hipStream_t library_stream()
+{
+    static hipStream_t s = []
+    {
+        hipStream_t st;
+        hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+        return st;
+    }();
+    return s;
+}
+
+std::string schedule_info::text() const
+{
+    std::string s = "gpu(gfx950): " + algorithm;
+    auto list     = [](const std::vector<int>& v)
+    {
+        std::string r;
+        for (size_t i = 0; i < v.size(); ++i)
+            r += (i ? "x" : "") + std::to_string(v[i]);
+        return r;
+    };
+    if (n1)
+        s += " n1=" + std::to_string(n1) + " n2=" + std::to_string(n2);
+    if (bluestein_m)
+        s += " m=" + std::to_string(bluestein_m);
+    s += " radices=" + list(radices);
+    if (threads_per_transform)
+        s += " threads/transform=" + std::to_string(threads_per_transform) +
+             " transforms/block=" + std::to_string(transforms_per_block);
+    return s;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Bluestein (the reference's dft_arblen_stage_impl, src/dft/dft-impl.hpp:199-271):
+// X = c . IFFT_M( FFT_M(x . c, zero padded) . conj(H) ), c[m] = exp(-i pi (m^2 mod 2n)/n),
+// H = FFT_M(h)/M with h the chirp mirrored into both ends of an M-point buffer. The inverse uses
+// conj(c) and H. M = next_pow2(2n - 1) as in the reference.
+
+// fused: both M-point transforms and the spectrum product in one workgroup, data in LDS throughout
+template <typename T, bool INV>
+__global__ void __launch_bounds__(1024) k_bluestein(lds_geom G, const cx<T>* __restrict__ tw, long long batch,
+                                                    const cx<T>* __restrict__ H, xload<T> ld, xstore<T> st,
+                                                    fastdiv mdiv)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* lds            = reinterpret_cast<cx<T>*>(smem_raw);
+    const long long first = (long long)blockIdx.x * G.tpb;
+    const fft_ctx<T> c    = make_ctx<T>(G, lds, first, batch);
+    block_load(G, lds, first, batch, ld, mdiv); // ld carries the chirp and the zero padding
+    __syncthreads();
+    fft_passes<T, false>(G, tw, c);
+    for (int i = c.t; i < G.n; i += G.tpt)
+        c.L[i * G.ls] = INV ? cmul(c.L[i * G.ls], H[i]) : cmulc(c.L[i * G.ls], H[i]);
+    __syncthreads();
+    fft_passes<T, true>(G, tw, c);
+    block_store(G, lds, first, batch, st, mdiv); // st carries the chirp and the truncation
+}
+
+// h: the FP64 chirp mirrored into both ends of M points
+__global__ void k_bluestein_h(cx<double>* h, const cx<double>* chirp, long long n, long long M)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M)
+        return;
+    cx<double> v = { 0.0, 0.0 };
+    if (i < n)
+        v = chirp[i];
+    else if (i > M - n)
+        v = chirp[M - i];
+    h[i] = v;
+}
+
+template <typename T>
+__global__ void k_convert_scale(cx<T>* out, const cx<double>* in, long long count, double scale)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count)
+        out[i] = { (T)(in[i].re * scale), (T)(in[i].im * scale) };
+}
+
+// ------------------------------------------------------------------------------------------------
+// complex plan
+
+template <typename T>
+struct gpu_dft<T>::impl
+{
+    enum algo_t
+    {
+        CORE,
+        BS_FUSED,
+        BS_CHAIN
+    } algo = CORE;
+    size_t n = 0;
+    schedule_info info;
+    std::unique_ptr<core_fft<T>> core; // length n (CORE) or M (BS_CHAIN)
+    size_t M = 0;
+    stockham bsS;
+    int bs_tpb = 1;
+    dev_mem bs_tw, chirp, H, Y;
+    long long bs_chunk = 1;
+
+    explicit impl(size_t len)
+    {
+        if (len == 0)
+            throw std::logic_error("DFT size must be positive");
+        n                 = len;
+        hipStream_t s     = library_stream();
+        const plan_desc d = describe_dft(sizeof(T) == 4 ? dtype_t::f32 : dtype_t::f64, n, false);
+        if (!d.arblen)
+        {
+            core.reset(new core_fft<T>(n, s));
+            info.algorithm = core->fourstep ? "four-step" : "lds";
+            info.radices   = core->radices();
+            info.n1        = core->n1;
+            info.n2        = core->n2;
+            if (!core->fourstep)
+            {
+                info.threads_per_transform = core->S.tpt;
+                info.transforms_per_block  = core->tpb;
+            }
+        }
+        else
+            init_bluestein(s);
+        hip_check(hipStreamSynchronize(s), "plan initialisation");
+    }
+
+    void init_bluestein(hipStream_t s)
+    {
+        M                = next_pow2(2 * n - 1);
+        info.bluestein_m = M;
+        dev_mem c64(n * sizeof(cx<double>));
+        gen_twiddles(c64.as<cx<double>>(), (long long)n, 2 * (long long)n, 0, 0, tw_square, 0, s);
+        chirp.reset(n * sizeof(cx<T>));
+        hipLaunchKernelGGL(k_convert_scale<T>, dim3(nblocks(n, 256)), dim3(256), 0, s, chirp.as<cx<T>>(),
+                           c64.as<cx<double>>(), (long long)n, 1.0);
+        hip_check(hipGetLastError(), "chirp conversion");
+        // H = FFT_M(h) / M evaluated in FP64 and rounded once
+        dev_mem h64(M * sizeof(cx<double>));
+        hipLaunchKernelGGL(k_bluestein_h, dim3(nblocks(M, 256)), dim3(256), 0, s, h64.as<cx<double>>(),
+                           c64.as<cx<double>>(), (long long)n, (long long)M);
+        hip_check(hipGetLastError(), "bluestein kernel spectrum");
+        {
+            core_fft<double> f64(M, s);
+            f64.run<false>(load_plain<double>(h64.ptr, 0), store_plain<double>(h64.ptr, 0), 1, s);
+            hip_check(hipStreamSynchronize(s), "bluestein kernel spectrum");
+        }
+        H.reset(M * sizeof(cx<T>));
+        hipLaunchKernelGGL(k_convert_scale<T>, dim3(nblocks(M, 256)), dim3(256), 0, s, H.as<cx<T>>(),
+                           h64.as<cx<double>>(), (long long)M, 1.0 / (double)M);
+        hip_check(hipGetLastError(), "bluestein kernel spectrum");
+        if (core_fft<T>::fits_lds(M))
+        {
+            algo = BS_FUSED;
+            make_stockham((long long)M, bsS);
+            bs_tpb = core_fft<T>::pick_tpb(bsS);
+            bs_tw.reset((size_t)std::max<long long>(1, bsS.tw_count) * sizeof(cx<T>));
+            gen_stockham_twiddles(bsS, bs_tw.as<cx<T>>(), s);
+            info.algorithm             = "bluestein-fused";
+            info.radices               = bsS.radix;
+            info.threads_per_transform = bsS.tpt;
+            info.transforms_per_block  = bs_tpb;
+        }
+        else
+        {
+            algo = BS_CHAIN;
+            core.reset(new core_fft<T>(M, s));
+            info.algorithm = "bluestein";
+            info.radices   = core->radices();
+            info.n1        = core->n1;
+            info.n2        = core->n2;
+            bs_chunk       = (long long)std::max<size_t>(1, ((size_t)256 << 20) / (M * sizeof(cx<T>)));
+        }
+        hip_check(hipStreamSynchronize(s), "bluestein initialisation");
+    }
+
+    template <bool INV>
+    void run_t(const xload<T>& ld, const xstore<T>& st, long long batch, hipStream_t s)
+    {
+        if (batch <= 0)
+            return;
+        if (algo == CORE)
+        {
+            core->template run<INV>(ld, st, batch, s);
+            return;
+        }
+        // the chirp post-multiplies the input and pre-multiplies the output (conjugated for the inverse)
+        xload<T> lc  = ld;
+        lc.mtab      = chirp.as<cx<T>>();
+        lc.mconj     = INV;
+        lc.lim       = (long long)n;
+        xstore<T> sc = st;
+        sc.mtab      = chirp.as<cx<T>>();
+        sc.mconj     = INV;
+        sc.lim       = (long long)n;
+        if (algo == BS_FUSED)
+        {
+            lds_geom G      = make_geom(bsS, bs_tpb, 0, 1, (int)M);
+            const size_t sh = (size_t)bs_tpb * M * sizeof(cx<T>);
+            allow_lds(k_bluestein<T, INV>, sh);
+            hipLaunchKernelGGL((k_bluestein<T, INV>), dim3(nblocks(batch, bs_tpb)), dim3(bsS.tpt * bs_tpb), sh, s, G,
+                               bs_tw.as<cx<T>>(), batch, H.as<cx<T>>(), lc, sc, make_fastdiv((uint32_t)M));
+            hip_check(hipGetLastError(), "bluestein launch");
+            return;
+        }
+        Y.ensure((size_t)std::min(batch, bs_chunk) * M * sizeof(cx<T>));
+        for (long long b0 = 0; b0 < batch; b0 += bs_chunk)
+        {
+            const long long nb = std::min(bs_chunk, batch - b0);
+            xload<T> l1        = lc;
+            l1.b0 += b0;
+            core->template run<false>(l1, store_plain<T>(Y.ptr, (long long)M), nb, s);
+            xload<T> l2  = load_plain<T>(Y.ptr, (long long)M);
+            l2.mode      = ld_mul;
+            l2.q         = H.as<cx<T>>();
+            l2.flag      = INV ? 0 : 1;
+            xstore<T> s2 = sc;
+            s2.b0 += b0;
+            core->template run<true>(l2, s2, nb, s);
+        }
+    }
+
+    void run(const xload<T>& ld, const xstore<T>& st, long long batch, bool inverse, hipStream_t s)
+    {
+        if (inverse)
+            run_t<true>(ld, st, batch, s);
+        else
+            run_t<false>(ld, st, batch, s);
+    }
+};
+
+template <typename T>
+gpu_dft<T>::gpu_dft(size_t n) : p(new impl(n))
+{
+}
+template <typename T>
+gpu_dft<T>::~gpu_dft() = default;
+template <typename T>
+size_t gpu_dft<T>::size() const
+{
+    return p->n;
+}
+template <typename T>
+const schedule_info& gpu_dft<T>::schedule() const
+{
+    return p->info;
+}
+
+template <typename T>
+void gpu_dft<T>::execute_dist(void* out, long long odist, const void* in, long long idist, long long batch,
+                              bool inverse, hipStream_t s)
+{
+    // In place is safe: the LDS kernel loads a block's transforms before storing them, the four-step
+    // column pass writes only the intermediate, and Bluestein stages through its own buffer.
+    p->run(load_plain<T>(in, idist), store_plain<T>(out, odist), batch, inverse, s);
+}
+
+template <typename T>
+void gpu_dft<T>::execute(void* out, const void* in, long long batch, bool inverse, hipStream_t s)
+{
+    execute_dist(out, (long long)p->n, in, (long long)p->n, batch, inverse, s);
+}
+
+// ------------------------------------------------------------------------------------------------
+// real plan (dft_plan_real): the n real values are read as n/2 complex values z, Z = FFT(z), and the
+// spectrum is repacked (to_fmt / from_fmt, src/dft/fft-impl.hpp:1799-1896) with the twiddles
+// tw[k] = exp(-i (2 pi k/n + pi/2)) = W_{4n}^{4k+n}.
+
+template <typename T>
+__device__ __forceinline__ void repack_pair(cx<T>* o, const cx<T>* __restrict__ rtw, long long csize, long long k,
+                                            cx<T> zk, cx<T> znk)
+{
+    // to_fmt (src/dft/fft-impl.hpp:1799-1841) for output pair (k, csize-k), 0 < k < (csize+1)/2
+    const cx<T> fpnk = cconj(znk);
+    const cx<T> f1   = cadd(zk, fpnk);
+    const cx<T> f2   = csub(zk, fpnk);
+    const cx<T> t    = cmul(f2, rtw[k]);
+    o[k]             = cscale(cadd(f1, t), T(0.5));
+    o[csize - k]     = cconj(cscale(csub(f1, t), T(0.5)));
+}
+template <typename T>
+__device__ __forceinline__ void repack_dc(cx<T>* o, long long csize, int fmt, cx<T> dc)
+{
+    if (fmt == 1)
+    {
+        o[0]     = { dc.re + dc.im, T(0) };
+        o[csize] = { dc.re - dc.im, T(0) };
+    }
+    else
+        o[0] = { dc.re + dc.im, dc.re - dc.im };
+}
+
+// repack of a computed half-length spectrum Z in global memory: one thread per pair
+template <typename T>
+__global__ void k_real_repack(cx<T>* out, long long odist, const cx<T>* Z, long long zdist,
+                              const cx<T>* __restrict__ rtw, long long csize, int fmt, long long batch)
+{
+    const long long count = (csize + 1) / 2;
+    const long long k     = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long b     = (long long)blockIdx.y + (long long)gridDim.y * blockIdx.z;
+    if (b >= batch || k > count)
+        return;
+    const cx<T>* z = Z + b * zdist;
+    cx<T>* o       = out + b * odist;
+    if (k == 0)
+        repack_dc(o, csize, fmt, z[0]);
+    else if (k == count)
+    {
+        if (csize % 2 == 0)
+            o[csize / 2] = cconj(z[csize / 2]);
+    }
+    else
+        repack_pair(o, rtw, csize, k, z[k], z[csize - k]);
+}
+
+// fused forward real transform for LDS-resident half lengths: the FFT result is repacked from LDS
+template <typename T>
+__global__ void __launch_bounds__(1024) k_real_fwd(lds_geom G, const cx<T>* __restrict__ tw, long long batch,
+                                                   xload<T> ld, cx<T>* out, long long odist,
+                                                   const cx<T>* __restrict__ rtw, int fmt, fastdiv ndiv)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* lds            = reinterpret_cast<cx<T>*>(smem_raw);
+    const long long first = (long long)blockIdx.x * G.tpb;
+    const fft_ctx<T> c    = make_ctx<T>(G, lds, first, batch);
+    block_load(G, lds, first, batch, ld, ndiv);
+    __syncthreads();
+    fft_passes<T, false>(G, tw, c);
+    if (!c.valid)
+        return;
+    const long long csize = G.n;
+    const long long count = (csize + 1) / 2;
+    cx<T>* o              = out + c.g * odist;
+    for (long long k = c.t; k <= count; k += G.tpt)
+    {
+        if (k == 0)
+            repack_dc(o, csize, fmt, c.L[0]);
+        else if (k == count)
+        {
+            if (csize % 2 == 0)
+                o[csize / 2] = cconj(c.L[(csize / 2) * G.ls]);
+        }
+        else
+            repack_pair(o, rtw, csize, k, c.L[k * G.ls], c.L[(csize - k) * G.ls]);
+    }
+}
+
+template <typename T>
+struct gpu_dft_real<T>::impl
+{
+    size_t n = 0, csize = 0;
+    int fmt = 1;
+    std::unique_ptr<typename gpu_dft<T>::impl> c; // complex n/2 plan
+    dev_mem rtw;
+    dev_mem Z;
+
+    impl(size_t len, int f)
+    {
+        if (len % 2 != 0 || len == 0)
+            throw std::logic_error("dft_plan_real requires size to be even");
+        n     = len;
+        csize = len / 2;
+        fmt   = f;
+        c.reset(new typename gpu_dft<T>::impl(csize));
+        hipStream_t s         = library_stream();
+        const long long count = (long long)(csize + 1) / 2 + 1;
+        rtw.reset((size_t)count * sizeof(cx<T>));
+        gen_twiddles(rtw.as<cx<T>>(), count, 4 * (long long)n, 4, (long long)n, tw_linear, 0, s);
+        hip_check(hipStreamSynchronize(s), "real plan initialisation");
+    }
+    size_t complex_size() const { return fmt == 1 ? csize + 1 : csize; }
+
+    bool fused_forward() const { return c->algo == gpu_dft<T>::impl::CORE && !c->core->fourstep; }
+
+    void forward_op(const xload<T>& ld, cx<T>* out, long long odist, long long batch, hipStream_t s)
+    {
+        if (batch <= 0)
+            return;
+        if (fused_forward())
+        {
+            core_fft<T>& f  = *c->core;
+            lds_geom G      = make_geom(f.S, f.tpb, 0, 1, (int)csize);
+            const size_t sh = (size_t)f.tpb * csize * sizeof(cx<T>);
+            allow_lds(k_real_fwd<T>, sh);
+            hipLaunchKernelGGL((k_real_fwd<T>), dim3(nblocks(batch, f.tpb)), dim3(f.S.tpt * f.tpb), sh, s, G,
+                               f.tw.template as<cx<T>>(), batch, ld, out, odist, rtw.template as<cx<T>>(), fmt,
+                               make_fastdiv((uint32_t)csize));
+            hip_check(hipGetLastError(), "real forward launch");
+            return;
+        }
+        const long long per   = (long long)csize;
+        const long long chunk = std::max<long long>(1, ((long long)256 << 20) / (per * (long long)sizeof(cx<T>)));
+        Z.ensure((size_t)std::min(chunk, batch) * per * sizeof(cx<T>));
+        for (long long b0 = 0; b0 < batch; b0 += chunk)
+        {
+            const long long nb = std::min(chunk, batch - b0);
+            xload<T> l         = ld;
+            l.b0 += b0;
+            c->run(l, store_plain<T>(Z.ptr, per), nb, false, s);
+            const long long count = (long long)(csize + 1) / 2;
+            dim3 grid(nblocks(count + 1, 256), (unsigned)std::min<long long>(nb, 65535),
+                      (unsigned)((nb + 65534) / 65535));
+            hipLaunchKernelGGL(k_real_repack<T>, grid, dim3(256), 0, s, out + b0 * odist, odist, Z.as<cx<T>>(), per,
+                               rtw.as<cx<T>>(), (long long)csize, fmt, nb);
+            hip_check(hipGetLastError(), "real repack launch");
+        }
+    }
+
+    // inverse: `src` describes the half spectrum (plain or a product of two spectra)
+    void inverse_op(xload<T> src, const xstore<T>& st, long long batch, hipStream_t s)
+    {
+        src.mode = ld_c2r;
+        src.r    = rtw.as<cx<T>>();
+        src.n    = (long long)csize;
+        src.flag = fmt;
+        c->run(src, st, batch, true, s);
+    }
+};
+
+template <typename T>
+gpu_dft_real<T>::gpu_dft_real(size_t n, int fmt) : p(new impl(n, fmt))
+{
+}
+template <typename T>
+gpu_dft_real<T>::~gpu_dft_real() = default;
+template <typename T>
+size_t gpu_dft_real<T>::size() const
+{
+    return p->n;
+}
+template <typename T>
+size_t gpu_dft_real<T>::complex_size() const
+{
+    return p->complex_size();
+}
+template <typename T>
+int gpu_dft_real<T>::format() const
+{
+    return p->fmt;
+}
+template <typename T>
+const schedule_info& gpu_dft_real<T>::schedule() const
+{
+    return p->c->info;
+}
+template <typename T>
+void gpu_dft_real<T>::forward(void* cout, const void* rin, long long batch, hipStream_t s)
+{
+    p->forward_op(load_plain<T>(rin, (long long)p->csize), static_cast<cx<T>*>(cout), (long long)p->complex_size(),
+                  batch, s);
+}
+template <typename T>
+void gpu_dft_real<T>::inverse(void* rout, const void* cin, long long batch, hipStream_t s)
+{
+    xload<T> src = load_plain<T>(cin, (long long)p->complex_size());
+    p->inverse_op(src, store_plain<T>(rout, (long long)p->csize), batch, s);
+}
+
+// ------------------------------------------------------------------------------------------------
+// DCT (dct_plan, include/kfr/dft/fft.hpp:927-970): DCT-II through an n-point complex DFT of the
+// even/odd mirrored input, y_k = Re(Y_k W_{4n}^k); the inverse (DCT-III) pre-multiplies by
+// W_{4n}^k with x_0 halved and de-interleaves the real parts.
+
+template <typename T>
+struct gpu_dct<T>::impl
+{
+    size_t n = 0;
+    std::unique_ptr<typename gpu_dft<T>::impl> c;
+    dev_mem w; // W_{4n}^k, k < n
+    explicit impl(size_t len)
+    {
+        n = len;
+        c.reset(new typename gpu_dft<T>::impl(n));
+        hipStream_t s = library_stream();
+        w.reset(std::max<size_t>(1, n) * sizeof(cx<T>));
+        gen_twiddles(w.as<cx<T>>(), (long long)n, 4 * (long long)n, 1, 0, tw_linear, 0, s);
+        hip_check(hipStreamSynchronize(s), "dct plan initialisation");
+    }
+};
+
+template <typename T>
+gpu_dct<T>::gpu_dct(size_t n) : p(new impl(n))
+{
+}
+template <typename T>
+gpu_dct<T>::~gpu_dct() = default;
+template <typename T>
+size_t gpu_dct<T>::size() const
+{
+    return p->n;
+}
+template <typename T>
+const schedule_info& gpu_dct<T>::schedule() const
+{
+    return p->c->info;
+}
+template <typename T>
+void gpu_dct<T>::execute(void* out, const void* in, long long batch, bool inverse, hipStream_t s)
+{
+    const long long n = (long long)p->n;
+    xload<T> l;
+    xstore<T> st;
+    l.p  = in;
+    l.n  = n;
+    st.p = out;
+    st.n = n;
+    if (!inverse)
+    {
+        l.mode  = ld_dct2;
+        st.mode = st_dct2;
+        st.q    = p->w.template as<cx<T>>();
+    }
+    else
+    {
+        l.mode  = ld_dct3;
+        l.q     = p->w.template as<cx<T>>();
+        st.mode = st_dct3;
+    }
+    p->c->run(l, st, batch, false, s);
+}
+
+// ------------------------------------------------------------------------------------------------
+// FFT convolution of real signals (convolve / correlate, src/dft/convolution-impl.cpp:106-128),
+// computed with a real FFT of S = next_pow2(nx + nh - 1) points, and its adjoint.
+
+template <typename T>
+inline xload<T> load_real_pad(const void* x, long long dist, long long nvalid, bool reverse)
+{
+    xload<T> l;
+    l.mode  = ld_real_pad;
+    l.p     = x;
+    l.pdist = dist;
+    l.n     = nvalid;
+    l.flag  = reverse ? 1 : 0;
+    return l;
+}
+template <typename T>
+inline xstore<T> store_real_trunc(void* y, long long dist, long long nvalid, T scale)
+{
+    xstore<T> s;
+    s.mode  = st_real_trunc;
+    s.p     = y;
+    s.pdist = dist;
+    s.n     = nvalid;
+    s.scale = scale;
+    return s;
+}
+// half spectrum a*b*scale (conj: a*conj(b)*scale)
+template <typename T>
+inline xload<T> spec_product(const void* a, long long adist, const cx<T>* b, long long bdist, T scale, bool conj)
+{
+    xload<T> l;
+    l.p     = a;
+    l.pdist = adist;
+    l.q     = b;
+    l.qdist = bdist;
+    l.scale = scale;
+    l.src   = conj ? 2 : 1;
+    return l;
+}
+
+template <typename T>
+struct gpu_conv<T>::impl
+{
+    size_t nx = 0, nh = 0, ny = 0, S = 0;
+    std::unique_ptr<typename gpu_dft_real<T>::impl> r;
+    dev_mem X, Hs, G, tmp;
+    impl(size_t a, size_t b)
+    {
+        if (a == 0 || b == 0)
+            throw std::logic_error("convolution inputs must not be empty");
+        nx = a;
+        nh = b;
+        ny = a + b - 1;
+        S  = std::max<size_t>(2, next_pow2(ny));
+        r.reset(new typename gpu_dft_real<T>::impl(S, 1));
+    }
+    long long cs() const { return (long long)(S / 2 + 1); }
+};
+
+template <typename T>
+gpu_conv<T>::gpu_conv(size_t nx, size_t nh) : p(new impl(nx, nh))
+{
+}
+template <typename T>
+gpu_conv<T>::~gpu_conv() = default;
+template <typename T>
+size_t gpu_conv<T>::fft_size() const
+{
+    return p->S;
+}
+template <typename T>
+size_t gpu_conv<T>::out_size() const
+{
+    return p->ny;
+}
+
+template <typename T>
+void gpu_conv<T>::forward(void* y, const void* x, const void* h, long long h_dist, long long batch, bool correlate,
+                          hipStream_t s)
+{
+    impl& q            = *p;
+    const long long cs = q.cs();
+    const long long hb = h_dist == 0 ? 1 : batch;
+    q.X.ensure((size_t)batch * cs * sizeof(cx<T>));
+    q.Hs.ensure((size_t)hb * cs * sizeof(cx<T>));
+    q.r->forward_op(load_real_pad<T>(x, (long long)q.nx, (long long)q.nx, false), q.X.template as<cx<T>>(), cs,
+                    batch, s);
+    q.r->forward_op(load_real_pad<T>(h, h_dist == 0 ? 0 : (long long)q.nh, (long long)q.nh, correlate),
+                    q.Hs.template as<cx<T>>(), cs, hb, s);
+    q.r->inverse_op(spec_product<T>(q.X.ptr, cs, q.Hs.template as<cx<T>>(), h_dist == 0 ? 0 : cs, T(1) / T(q.S), false),
+                    store_real_trunc<T>(y, (long long)q.ny, (long long)q.ny, T(1)), batch, s);
+}
+
+template <typename T>
+__global__ void k_sum_batch(T* out, const T* in, long long n, long long batch)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    T acc = T(0);
+    for (long long b = 0; b < batch; ++b)
+        acc += in[b * n + i];
+    out[i] = acc;
+}
+
+template <typename T>
+void gpu_conv<T>::backward(void* dx, void* dh, const void* g, const void* x, const void* h, long long h_dist,
+                           long long batch, hipStream_t s)
+{
+    impl& q            = *p;
+    const long long cs = q.cs();
+    const long long hb = h_dist == 0 ? 1 : batch;
+    q.G.ensure((size_t)batch * cs * sizeof(cx<T>));
+    q.Hs.ensure((size_t)hb * cs * sizeof(cx<T>));
+    const T scale = T(1) / T(q.S);
+    q.r->forward_op(load_real_pad<T>(g, (long long)q.ny, (long long)q.ny, false), q.G.template as<cx<T>>(), cs, batch,
+                    s);
+    if (dx)
+    {
+        // dx = g correlated with h: IFFT(G conj(H)), first nx samples
+        q.r->forward_op(load_real_pad<T>(h, h_dist == 0 ? 0 : (long long)q.nh, (long long)q.nh, false),
+                        q.Hs.template as<cx<T>>(), cs, hb, s);
+        q.r->inverse_op(spec_product<T>(q.G.ptr, cs, q.Hs.template as<cx<T>>(), h_dist == 0 ? 0 : cs, scale, true),
+                        store_real_trunc<T>(dx, (long long)q.nx, (long long)q.nx, T(1)), batch, s);
+    }
+    if (dh)
+    {
+        // dh = g correlated with x: IFFT(G conj(X)), first nh samples; X is recomputed here
+        q.X.ensure((size_t)batch * cs * sizeof(cx<T>));
+        q.r->forward_op(load_real_pad<T>(x, (long long)q.nx, (long long)q.nx, false), q.X.template as<cx<T>>(), cs,
+                        batch, s);
+        if (h_dist != 0)
+            q.r->inverse_op(spec_product<T>(q.G.ptr, cs, q.X.template as<cx<T>>(), cs, scale, true),
+                            store_real_trunc<T>(dh, (long long)q.nh, (long long)q.nh, T(1)), batch, s);
+        else
+        {
+            // shared taps: the gradient sums over the batch
+            q.tmp.ensure((size_t)batch * q.nh * sizeof(T));
+            q.r->inverse_op(spec_product<T>(q.G.ptr, cs, q.X.template as<cx<T>>(), cs, scale, true),
+                            store_real_trunc<T>(q.tmp.ptr, (long long)q.nh, (long long)q.nh, T(1)), batch, s);
+            hipLaunchKernelGGL(k_sum_batch<T>, dim3(nblocks((long long)q.nh, 256)), dim3(256), 0, s,
+                               static_cast<T*>(dh), q.tmp.template as<T>(), (long long)q.nh, batch);
+            hip_check(hipGetLastError(), "gradient reduction");
+        }
+    }
+}
+
+// complex convolution: complex FFT of S points; keep_imag = false returns the real part as the
+// reference does (src/dft/convolution-impl.cpp:124-127)
+template <typename T>
+void gpu_convolve_complex(void* y, const void* a, size_t na, const void* b, size_t nb, bool correlate,
+                          bool keep_imag, hipStream_t s)
+{
+    const size_t ny = na + nb - 1;
+    const size_t S  = next_pow2(ny);
+    typename gpu_dft<T>::impl f(S);
+    dev_mem A(S * sizeof(cx<T>)), B(S * sizeof(cx<T>));
+    xload<T> la;
+    la.mode = ld_cpad;
+    la.p    = a;
+    la.n    = (long long)na;
+    f.run(la, store_plain<T>(A.ptr, 0), 1, false, s);
+    xload<T> lb = la;
+    lb.p        = b;
+    lb.n        = (long long)nb;
+    lb.flag     = correlate ? 1 : 0;
+    f.run(lb, store_plain<T>(B.ptr, 0), 1, false, s);
+    xload<T> lm = load_plain<T>(A.ptr, 0);
+    lm.mode     = ld_mul;
+    lm.q        = B.as<cx<T>>();
+    xstore<T> sy;
+    sy.mode  = st_ctrunc;
+    sy.p     = y;
+    sy.n     = (long long)ny;
+    sy.scale = T(1) / T(S);
+    sy.flag  = keep_imag ? 1 : 0;
+    f.run(lm, sy, 1, true, s);
+    hip_check(hipStreamSynchronize(s), "complex convolution");
+}
+
+// ------------------------------------------------------------------------------------------------
+// Streaming convolution filter (convolve_filter<T>, src/dft/convolution-impl.cpp:37-228). The
+// reference's uniformly partitioned overlap-add produces y[t] = sum_k h[k] x[t-k] across calls. Here
+// each call is one overlap-save batch: the call's input is appended to the last ntaps-1 samples of
+// history, cut into segments of S points overlapping by ntaps-1, and all segments are filtered by
+// one batched real FFT, a spectrum product fused into the inverse FFT's load, and an inverse FFT;
+// the last S-(ntaps-1) samples of each segment are the output.
+
+template <typename T>
+__global__ void k_gather_segments(T* seg, const T* hist, const T* in, long long nh1, long long n, long long S,
+                                  long long L, long long nseg)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nseg * S)
+        return;
+    const long long c = idx / S, j = idx - c * S;
+    const long long pos = c * L + j; // position in [history | input]
+    T v                 = T(0);
+    if (pos < nh1)
+        v = hist[pos];
+    else if (pos - nh1 < n)
+        v = in[pos - nh1];
+    seg[idx] = v;
+}
+
+template <typename T>
+__global__ void k_scatter_valid(T* out, const T* y, long long nh1, long long n, long long S, long long L,
+                                long long nseg)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nseg * L)
+        return;
+    const long long c = idx / L, j = idx - c * L;
+    const long long t = c * L + j;
+    if (t < n)
+        out[t] = y[c * S + nh1 + j];
+}
+
+template <typename T>
+__global__ void k_update_history(T* nhist, const T* hist, const T* in, long long nh1, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nh1)
+        return;
+    const long long pos = n + i; // position in [history | input]
+    nhist[i]            = pos < nh1 ? hist[pos] : in[pos - nh1];
+}
+
+template <typename T>
+__global__ void k_zero(T* p, long long count)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count)
+        p[i] = T(0);
+}
+
+template <typename T>
+struct gpu_conv_filter<T>::impl
+{
+    size_t block = 1024, ntaps = 0, nh1 = 0, S = 0, L = 0;
+    std::unique_ptr<typename gpu_dft_real<T>::impl> r;
+    dev_mem H, hist[2], seg, spec, yseg, taps_dev;
+    int cur = 0;
+
+    impl(const T* taps, size_t nt, size_t bs)
+    {
+        block = next_pow2(bs == 0 ? 1024 : bs);
+        set(taps, nt);
+    }
+    void set(const T* taps, size_t nt)
+    {
+        if (nt == 0)
+            throw std::logic_error("convolution filter needs at least one tap");
+        ntaps = nt;
+        nh1   = nt - 1;
+        S     = std::max<size_t>(next_pow2(std::max<size_t>(2 * block, 2 * nh1 + 2)), 2);
+        L     = S - nh1;
+        r.reset(new typename gpu_dft_real<T>::impl(S, 1));
+        hipStream_t s = library_stream();
+        taps_dev.reset(nt * sizeof(T));
+        hip_check(hipMemcpy(taps_dev.ptr, taps, nt * sizeof(T), hipMemcpyHostToDevice), "filter taps upload");
+        H.reset((S / 2 + 1) * sizeof(cx<T>));
+        r->forward_op(load_real_pad<T>(taps_dev.ptr, 0, (long long)nt, false), H.as<cx<T>>(), 0, 1, s);
+        hist[0].reset(std::max<size_t>(1, nh1) * sizeof(T));
+        hist[1].reset(std::max<size_t>(1, nh1) * sizeof(T));
+        hip_check(hipStreamSynchronize(s), "filter spectrum");
+        reset();
+    }
+    void reset()
+    {
+        hipStream_t s = library_stream();
+        for (auto& h : hist)
+        {
+            const long long cnt = (long long)(h.bytes / sizeof(T));
+            hipLaunchKernelGGL(k_zero<T>, dim3(nblocks(cnt, 256)), dim3(256), 0, s, h.template as<T>(), cnt);
+            hip_check(hipGetLastError(), "filter reset");
+        }
+        hip_check(hipStreamSynchronize(s), "filter reset");
+        cur = 0;
+    }
+    void process(T* out, const T* in, size_t n, hipStream_t s)
+    {
+        if (n == 0)
+            return;
+        const long long nseg = (long long)((n + L - 1) / L);
+        const long long cs   = (long long)(S / 2 + 1);
+        seg.ensure((size_t)nseg * S * sizeof(T));
+        spec.ensure((size_t)nseg * cs * sizeof(cx<T>));
+        yseg.ensure((size_t)nseg * S * sizeof(T));
+        const T* h0 = hist[cur].template as<T>();
+        hipLaunchKernelGGL(k_gather_segments<T>, dim3(nblocks(nseg * (long long)S, 256)), dim3(256), 0, s,
+                           seg.template as<T>(), h0, in, (long long)nh1, (long long)n, (long long)S, (long long)L,
+                           nseg);
+        hip_check(hipGetLastError(), "filter gather");
+        r->forward_op(load_plain<T>(seg.ptr, (long long)S / 2), spec.template as<cx<T>>(), cs, nseg, s);
+        r->inverse_op(spec_product<T>(spec.ptr, cs, H.template as<cx<T>>(), 0, T(1) / T(S), false),
+                      store_plain<T>(yseg.ptr, (long long)S / 2), nseg, s);
+        hipLaunchKernelGGL(k_scatter_valid<T>, dim3(nblocks(nseg * (long long)L, 256)), dim3(256), 0, s, out,
+                           yseg.template as<T>(), (long long)nh1, (long long)n, (long long)S, (long long)L, nseg);
+        hip_check(hipGetLastError(), "filter scatter");
+        if (nh1 > 0)
+        {
+            hipLaunchKernelGGL(k_update_history<T>, dim3(nblocks((long long)nh1, 256)), dim3(256), 0, s,
+                               hist[1 - cur].template as<T>(), h0, in, (long long)nh1, (long long)n);
+            hip_check(hipGetLastError(), "filter history");
+            cur = 1 - cur;
+        }
+    }
+};
+
+template <typename T>
+gpu_conv_filter<T>::gpu_conv_filter(const T* taps_host, size_t ntaps, size_t block_size)
+    : p(new impl(taps_host, ntaps, block_size))
+{
+}
+template <typename T>
+gpu_conv_filter<T>::~gpu_conv_filter() = default;
+template <typename T>
+size_t gpu_conv_filter<T>::block_size() const
+{
+    return p->block;
+}
+template <typename T>
+void gpu_conv_filter<T>::set_data(const T* taps_host, size_t ntaps)
+{
+    p->set(taps_host, ntaps);
+}
+template <typename T>
+void gpu_conv_filter<T>::reset()
+{
+    p->reset();
+}
+template <typename T>
+void gpu_conv_filter<T>::process(T* out, const T* in, size_t n, hipStream_t s)
+{
+    p->process(out, in, n, s);
+}
+
+// ------------------------------------------------------------------------------------------------
+// misc
+
+template <typename T>
+__global__ void k_scale(T* p, long long count, T f)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count)
+        p[i] *= f;
+}
+
+template <typename T>
+void gpu_scale(void* data, size_t count, T factor, hipStream_t s)
+{
+    if (!count)
+        return;
+    hipLaunchKernelGGL(k_scale<T>, dim3(nblocks((long long)count, 256)), dim3(256), 0, s, static_cast<T*>(data),
+                       (long long)count, factor);
+    hip_check(hipGetLastError(), "scale launch");
+}
+
+template class gpu_dft<float>;
+template class gpu_dft<double>;
+template class gpu_dft_real<float>;
+template class gpu_dft_real<double>;
+template class gpu_dct<float>;
+template class gpu_dct<double>;
+template class gpu_conv<float>;
+template class gpu_conv<double>;
+template class gpu_conv_filter<float>;
+template class gpu_conv_filter<double>;
+template void gpu_convolve_complex<float>(void*, const void*, size_t, const void*, size_t, bool, bool, hipStream_t);
+template void gpu_convolve_complex<double>(void*, const void*, size_t, const void*, size_t, bool, bool,
+                                           hipStream_t);
+template void gpu_scale<float>(void*, size_t, float, hipStream_t);
+template void gpu_scale<double>(void*, size_t, double, hipStream_t);
+
+} // namespace kfr_amd

@@ -1,0 +1,342 @@
+"""Python mirror of the reference's DFT class API over the C ABI.
+
+Names, argument meaning and error behaviour follow include/kfr/dft/fft.hpp and
+include/kfr/dft/convolution.hpp: dft_plan, dft_plan_real, dct_plan, convolve, correlate,
+autocorrelate, convolve_filter and dft_cache. Arrays may be numpy arrays (host memory, copied to the
+GPU and back) or torch tensors on the GPU (used in place). No scaling is applied by any transform.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import threading
+
+import numpy as np
+
+from ._lib import KfrError, check, lib
+
+CCs = 1
+Perm = 0
+
+
+def _ptr(a):
+    """Data pointer of a numpy array or a torch tensor."""
+    if isinstance(a, np.ndarray):
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("array must be C-contiguous")
+        return a.ctypes.data
+    if hasattr(a, "data_ptr"):
+        if not a.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        return a.data_ptr()
+    raise TypeError(f"unsupported array type {type(a)!r}")
+
+
+def _nbytes(a):
+    if isinstance(a, np.ndarray):
+        return a.nbytes
+    return a.numel() * a.element_size()
+
+
+def _tag(dtype) -> str:
+    dt = np.dtype(dtype)
+    if dt in (np.float32, np.complex64):
+        return "f32"
+    if dt in (np.float64, np.complex128):
+        return "f64"
+    raise TypeError("only float32/float64 (and their complex types) are supported")
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        return None
+    return getattr(stream, "cuda_stream", stream)
+
+
+class dft_plan:
+    """Complex DFT of `size` points (include/kfr/dft/fft.hpp:183-456)."""
+
+    def __init__(self, size: int, dtype=np.complex64, progressive_optimized: bool = False):
+        self.t = _tag(dtype)
+        self._progressive_optimized = progressive_optimized
+        L = lib()
+        self._h = getattr(L, f"kfr_dft_create_plan_{self.t}")(size)
+        if not self._h:
+            raise KfrError(L.kfr_last_error().decode())
+        self.size = getattr(L, f"kfr_dft_get_size_{self.t}")(self._h)
+        self.temp_size = getattr(L, f"kfr_dft_get_temp_size_{self.t}")(self._h)
+
+    @property
+    def complex_dtype(self):
+        return np.complex64 if self.t == "f32" else np.complex128
+
+    def execute(self, out, in_, temp=None, inverse: bool = False):
+        fn = f"kfr_dft_execute_inverse_{self.t}" if inverse else f"kfr_dft_execute_{self.t}"
+        getattr(lib(), fn)(self._h, _ptr(out), _ptr(in_), None if temp is None else _ptr(temp))
+        check()
+        return out
+
+    def execute_batch(self, out, in_, batch: int, inverse: bool = False, stream=None):
+        """Batched transform of device tensors (contiguous items of `size` points), asynchronous."""
+        r = getattr(lib(), f"kfr_amd_dft_execute_batch_{self.t}")(self._h, _ptr(out), _ptr(in_), batch,
+                                                                   int(inverse), _stream_ptr(stream))
+        check(r == 0)
+        return out
+
+    def dump(self):
+        getattr(lib(), f"kfr_dft_dump_{self.t}")(self._h)
+
+    def describe(self) -> str:
+        return describe(3 if self._progressive_optimized else 0, self.t, self.size)
+
+    def progressive_total_steps(self) -> int:
+        return lib().kfr_amd_dft_progressive_total_steps(0 if self.t == "f32" else 1, self.size)
+
+    # progressive execution (include/kfr/dft/fft.hpp:393-439): the result is complete in `out` after
+    # the last step; earlier steps only advance the counter.
+    def progressive_start(self, inverse, out, in_, temp=None):
+        if self.temp_size and temp is None:
+            raise KfrError("Temporary buffer must be provided for progressive execution")
+        return {"inverse": bool(inverse), "out": out, "in": in_, "temp": temp, "step": 0}
+
+    def progressive_step(self, prog) -> bool:
+        total = self.progressive_total_steps()
+        if prog["step"] == total - 1:
+            self.execute(prog["out"], prog["in"], prog["temp"], prog["inverse"])
+        prog["step"] += 1
+        return prog["step"] < total
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            getattr(lib(), f"kfr_dft_delete_plan_{self.t}")(h)
+            self._h = None
+
+
+class dft_plan_real:
+    """Real DFT of `size` (even) points (include/kfr/dft/fft.hpp:459-566)."""
+
+    def __init__(self, size: int, fmt: int = CCs, dtype=np.float32):
+        self.t = _tag(dtype)
+        L = lib()
+        self._h = getattr(L, f"kfr_dft_real_create_plan_{self.t}")(size, fmt)
+        if not self._h:
+            raise KfrError(L.kfr_last_error().decode())
+        self.size = getattr(L, f"kfr_dft_real_get_size_{self.t}")(self._h)
+        self.temp_size = getattr(L, f"kfr_dft_real_get_temp_size_{self.t}")(self._h)
+        self.fmt = fmt
+
+    def complex_size(self) -> int:
+        return self.size // 2 + 1 if self.fmt == CCs else self.size // 2
+
+    def execute(self, out, in_, temp=None):
+        """Forward when `in_` is real (writes complex_size() values), inverse when it is complex."""
+        is_complex = np.dtype(in_.dtype).kind == "c" if isinstance(in_, np.ndarray) else in_.is_complex()
+        fn = f"kfr_dft_real_execute_inverse_{self.t}" if is_complex else f"kfr_dft_real_execute_{self.t}"
+        getattr(lib(), fn)(self._h, _ptr(out), _ptr(in_), None if temp is None else _ptr(temp))
+        check()
+        return out
+
+    def execute_batch(self, out, in_, batch: int, inverse: bool = False, stream=None):
+        r = getattr(lib(), f"kfr_amd_dft_real_execute_batch_{self.t}")(self._h, _ptr(out), _ptr(in_), batch,
+                                                                        int(inverse), _stream_ptr(stream))
+        check(r == 0)
+        return out
+
+    def dump(self):
+        getattr(lib(), f"kfr_dft_real_dump_{self.t}")(self._h)
+
+    def describe(self) -> str:
+        return describe(1, self.t, self.size, self.fmt)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            getattr(lib(), f"kfr_dft_real_delete_plan_{self.t}")(h)
+            self._h = None
+
+
+class dct_plan:
+    """DCT-II (inverse: DCT-III with x[0] halved), unscaled (include/kfr/dft/fft.hpp:927-978)."""
+
+    def __init__(self, size: int, dtype=np.float32):
+        self.t = _tag(dtype)
+        L = lib()
+        self._h = getattr(L, f"kfr_dct_create_plan_{self.t}")(size)
+        if not self._h:
+            raise KfrError(L.kfr_last_error().decode())
+        self.size = getattr(L, f"kfr_dct_get_size_{self.t}")(self._h)
+        self.temp_size = getattr(L, f"kfr_dct_get_temp_size_{self.t}")(self._h)
+
+    def execute(self, out, in_, temp=None, inverse: bool = False):
+        fn = f"kfr_dct_execute_inverse_{self.t}" if inverse else f"kfr_dct_execute_{self.t}"
+        getattr(lib(), fn)(self._h, _ptr(out), _ptr(in_), None if temp is None else _ptr(temp))
+        check()
+        return out
+
+    def execute_batch(self, out, in_, batch: int, inverse: bool = False, stream=None):
+        r = getattr(lib(), f"kfr_amd_dct_execute_batch_{self.t}")(self._h, _ptr(out), _ptr(in_), batch,
+                                                                  int(inverse), _stream_ptr(stream))
+        check(r == 0)
+        return out
+
+    def dump(self):
+        getattr(lib(), f"kfr_dct_dump_{self.t}")(self._h)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            getattr(lib(), f"kfr_dct_delete_plan_{self.t}")(h)
+            self._h = None
+
+
+def _conv(mode: int, a: np.ndarray, b: np.ndarray | None):
+    a = np.ascontiguousarray(a)
+    dt = {np.dtype(np.float32): 0, np.dtype(np.float64): 1, np.dtype(np.complex64): 2,
+          np.dtype(np.complex128): 3}[a.dtype]
+    if b is None:
+        b = a
+    b = np.ascontiguousarray(b, dtype=a.dtype)
+    L = lib()
+    n = L.kfr_amd_convolve(dt, mode, a.ctypes.data, a.size, b.ctypes.data, b.size, None)
+    check(n >= 0)
+    out = np.empty(n, dtype=a.dtype)
+    r = L.kfr_amd_convolve(dt, mode, a.ctypes.data, a.size, b.ctypes.data, b.size, out.ctypes.data)
+    check(r >= 0)
+    return out
+
+
+def convolve(a, b):
+    """Linear convolution, len(a)+len(b)-1 values (include/kfr/dft/convolution.hpp:53-59)."""
+    return _conv(0, a, b)
+
+
+def correlate(a, b):
+    """Correlation: convolution with b reversed (include/kfr/dft/convolution.hpp:62-67)."""
+    return _conv(1, a, b)
+
+
+def autocorrelate(a):
+    """Second half of correlate(a, a) (include/kfr/dft/convolution.hpp:70-76)."""
+    return _conv(2, a, None)
+
+
+class convolve_filter:
+    """Streaming FFT convolution filter (include/kfr/dft/convolution.hpp:99-148), real types."""
+
+    def __init__(self, taps, block_size: int = 1024):
+        taps = np.ascontiguousarray(taps)
+        self.t = _tag(taps.dtype)
+        self.dtype = taps.dtype
+        L = lib()
+        self._h = getattr(L, f"kfr_filter_create_convolution_plan_{self.t}")(taps.ctypes.data, taps.size,
+                                                                             block_size)
+        if not self._h:
+            raise KfrError(L.kfr_last_error().decode())
+        b = 1
+        while b < (block_size or 1024):
+            b <<= 1
+        self.block_size = b
+
+    def input_block_size(self) -> int:
+        return self.block_size
+
+    def apply(self, dest, src=None):
+        if src is None:
+            src = dest
+        n = min(dest.size if isinstance(dest, np.ndarray) else dest.numel(),
+                src.size if isinstance(src, np.ndarray) else src.numel())
+        getattr(lib(), f"kfr_filter_process_{self.t}")(self._h, _ptr(dest), _ptr(src), n)
+        check()
+        return dest
+
+    def reset(self):
+        getattr(lib(), f"kfr_filter_reset_{self.t}")(self._h)
+        check()
+
+    def set_data(self, taps):
+        """Replace the filter taps (and reset the state)."""
+        taps = np.ascontiguousarray(taps, dtype=self.dtype)
+        getattr(lib(), f"kfr_filter_delete_plan_{self.t}")(self._h)
+        self._h = getattr(lib(), f"kfr_filter_create_convolution_plan_{self.t}")(taps.ctypes.data, taps.size,
+                                                                                 self.block_size)
+        if not self._h:
+            raise KfrError(lib().kfr_last_error().decode())
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            getattr(lib(), f"kfr_filter_delete_plan_{self.t}")(h)
+            self._h = None
+
+
+class dft_cache:
+    """Process-wide cache of complex plans (include/kfr/dft/cache.hpp:44-126). The key includes the
+    element type and the size, as the reference's does."""
+
+    _inst = None
+    _lock = threading.Lock()
+
+    def __init__(self):
+        self._plans = {}
+
+    @classmethod
+    def instance(cls) -> "dft_cache":
+        with cls._lock:
+            if cls._inst is None:
+                cls._inst = dft_cache()
+            return cls._inst
+
+    def get(self, dtype, size: int) -> dft_plan:
+        key = (_tag(dtype), int(size))
+        with self._lock:
+            p = self._plans.get(key)
+            if p is None:
+                p = dft_plan(size, np.complex64 if key[0] == "f32" else np.complex128)
+                self._plans[key] = p
+            return p
+
+    def clear(self):
+        with self._lock:
+            self._plans.clear()
+
+
+def dft(x: np.ndarray) -> np.ndarray:
+    """Forward DFT through the cache (include/kfr/dft/cache.hpp:128-139)."""
+    x = np.ascontiguousarray(x)
+    p = dft_cache.instance().get(x.dtype, x.size)
+    out = np.empty_like(x)
+    return p.execute(out, x)
+
+
+def idft(x: np.ndarray) -> np.ndarray:
+    """Inverse DFT through the cache, unscaled (include/kfr/dft/cache.hpp:140-151)."""
+    x = np.ascontiguousarray(x)
+    p = dft_cache.instance().get(x.dtype, x.size)
+    out = np.empty_like(x)
+    return p.execute(out, x, inverse=True)
+
+
+def describe(kind: int, t: str, size: int, arg: int = 0) -> str:
+    """The reference plan descriptor (dump() layout), computed on the host."""
+    buf = C.create_string_buffer(1 << 16)
+    n = lib().kfr_amd_describe(kind, 0 if t in ("f32", 0) else 1, size, arg, buf, len(buf))
+    check(n >= 0)
+    if n >= len(buf):
+        buf = C.create_string_buffer(n + 1)
+        lib().kfr_amd_describe(kind, 0 if t in ("f32", 0) else 1, size, arg, buf, len(buf))
+    return buf.value.decode()
+
+
+def permutation(t: str, size: int) -> np.ndarray | None:
+    """Index array of the reference plan's reorder stage (out[i] = in[perm[i]]), or None."""
+    perm = np.empty(size, dtype=np.int64)
+    r = lib().kfr_amd_dft_permutation(0 if t == "f32" else 1, size, perm.ctypes.data)
+    check(r >= 0)
+    return perm if r == 1 else None
+
+
+def schedule(t: str, size: int) -> str:
+    buf = C.create_string_buffer(4096)
+    r = lib().kfr_amd_dft_schedule(0 if t == "f32" else 1, size, buf, len(buf))
+    check(r >= 0)
+    return buf.value.decode()
