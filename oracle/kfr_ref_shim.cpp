@@ -22,6 +22,8 @@
 #include <thread>
 #include <vector>
 
+#include <unistd.h>
+
 using namespace kfr;
 
 #define KREF_API extern "C" __attribute__((visibility("default")))
@@ -38,13 +40,22 @@ std::string describe_stages(const dft_plan<T>& plan)
                   plan.size, plan.temp_size, plan.data_size, plan.stages[0].size(), plan.stages[1].size(),
                   (int)plan.arblen);
     s += line;
-    for (const auto& st : plan.all_stages)
+    // the stage lines are the reference's own dump() output, captured from stdout
     {
-        std::snprintf(line, sizeof(line), "%s: %zu, %zu, %zu, %zu, %zu, %zu, %zu, %d, %d\n",
-                      st->name ? st->name : "unnamed", st->radix, st->stage_size, st->data_size,
-                      st->temp_size, st->repeats, st->out_offset, st->blocks, (int)st->recursion,
-                      (int)st->can_inplace);
-        s += line;
+        std::fflush(stdout);
+        FILE* tf        = std::tmpfile();
+        const int saved = dup(1);
+        dup2(fileno(tf), 1);
+        plan.dump();
+        std::fflush(stdout);
+        dup2(saved, 1);
+        close(saved);
+        std::rewind(tf);
+        char buf[4096];
+        size_t got;
+        while ((got = std::fread(buf, 1, sizeof(buf), tf)) > 0)
+            s.append(buf, got);
+        std::fclose(tf);
     }
     // stage order of the forward and inverse lists, as indexes into all_stages
     for (int dir = 0; dir < 2; ++dir)
