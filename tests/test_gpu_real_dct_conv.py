@@ -1,0 +1,215 @@
+"""Parity of the GPU real DFT, DCT, convolution, streaming filter and convolution adjoint (gpu)."""
+from __future__ import annotations
+
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import kfr_amd
+import oracle
+from tests.common import fp64_dct, fp64_rdft, tol_kfr
+
+pytestmark = pytest.mark.gpu
+GOLD = Path(__file__).resolve().parent / "golden"
+KAT = json.loads((GOLD / "kat.json").read_text())
+SP = np.load(GOLD / "spectra.npz")
+
+
+@pytest.mark.parametrize("n", [2, 4, 6, 16, 30, 64, 202, 2048, 4096, 30720, 1 << 15, 1 << 17, 6326, 1 << 20])
+@pytest.mark.parametrize("fmt", [0, 1])
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_real_dft(n, fmt, dt):
+    x = oracle.gaussian(700 + n, n, dt)
+    p = kfr_amd.dft_plan_real(n, fmt, dt)
+    cdt = np.complex64 if dt == np.float32 else np.complex128
+    X = np.empty(p.complex_size(), dtype=cdt)
+    p.execute(X, x)
+    ref = fp64_rdft(x.astype(np.float64), fmt)
+    e = oracle.rel_rms(X, ref)
+    bound = max(1e-6 if dt == np.float32 else 1e-14, tol_kfr(n, dt))
+    print(f"real n={n} fmt={fmt} {np.dtype(dt).name} fwd rel_rms={e:.3e}", flush=True)
+    assert e <= bound
+    y = np.empty(n, dtype=dt)
+    p.execute(y, X)
+    e2 = oracle.rel_rms(y / n, x)
+    assert e2 <= bound * 2, e2
+
+
+@pytest.mark.parametrize("tn", ["f32", "f64"])
+def test_real_dft_matches_reference_outputs(tn):
+    dt = np.float32 if tn == "f32" else np.float64
+    for n in [2, 4, 30, 64, 2048]:
+        x = SP[f"r_in_{tn}_{n}"]
+        for fmt in (0, 1):
+            p = kfr_amd.dft_plan_real(n, fmt, dt)
+            X = np.empty(p.complex_size(), dtype=SP[f"r_fwd_{tn}_{n}_{fmt}"].dtype)
+            p.execute(X, x)
+            assert oracle.rel_rms(X, SP[f"r_fwd_{tn}_{n}_{fmt}"]) <= 2 * tol_kfr(n, dt)
+            y = np.empty(n, dtype=dt)
+            p.execute(y, SP[f"r_fwd_{tn}_{n}_{fmt}"])
+            assert oracle.rel_rms(y, SP[f"r_inv_{tn}_{n}_{fmt}"]) <= 2 * tol_kfr(n, dt)
+
+
+def test_dct_known_answers():
+    p = kfr_amd.dct_plan(16, np.float32)
+    x = np.arange(16, dtype=np.float32)
+    y = np.empty(16, np.float32)
+    p.execute(y, x)
+    assert np.sqrt(np.mean((y - KAT["dct_fwd"]) ** 2)) < 1e-5
+    p.execute(y, x, inverse=True)
+    assert np.sqrt(np.mean((y - KAT["dct_inv"]) ** 2)) < 1e-5
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 16, 17, 64, 1000, 4096, 65536])
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_dct(n, dt):
+    x = oracle.gaussian(900 + n, n, dt)
+    p = kfr_amd.dct_plan(n, dt)
+    for inv in (False, True):
+        y = np.empty(n, dtype=dt)
+        p.execute(y, x, inverse=inv)
+        ref = fp64_dct(x.astype(np.float64), inv)
+        e = oracle.rel_rms(y, ref)
+        assert e <= max(2e-6 if dt == np.float32 else 1e-13, tol_kfr(n, dt)), (n, inv, e)
+
+
+def test_dct_batch_65536_x_1024_sample():
+    torch = pytest.importorskip("torch")
+    n, b = 65536, 64
+    x = oracle.gaussian(42, n * b, np.float32)
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.empty_like(xd)
+    p = kfr_amd.dct_plan(n, np.float32)
+    p.execute_batch(yd, xd, b)
+    torch.cuda.synchronize()
+    y = yd.cpu().numpy().reshape(b, n)
+    ref = fp64_dct(x.reshape(b, n)[:4].astype(np.float64))
+    assert max(oracle.rel_rms(y[i], ref[i]) for i in range(4)) < 2e-6
+
+
+def test_convolution_known_answers():
+    for dt in (np.float32, np.float64):
+        a = np.array(KAT["conv_a"], dtype=dt)
+        b = np.array(KAT["conv_b"], dtype=dt)
+        assert np.sqrt(np.mean((kfr_amd.convolve(a, b) - KAT["convolve"]) ** 2)) < 1e-4
+        assert np.sqrt(np.mean((kfr_amd.correlate(a, b) - KAT["correlate"]) ** 2)) < 1e-4
+    ac = np.array(KAT["conv_a"], dtype=np.complex64)
+    bc = np.array(KAT["conv_b"], dtype=np.complex64)
+    assert np.sqrt(np.mean(np.abs(kfr_amd.convolve(ac, bc) - KAT["convolve"]) ** 2)) < 1e-4
+
+
+@pytest.mark.parametrize("tn", ["f32", "f64"])
+def test_convolution_matches_reference_outputs(tn):
+    a, b = SP[f"conv_a_{tn}"], SP[f"conv_b_{tn}"]
+    tol = 1e-5 if tn == "f32" else 1e-12
+    assert oracle.rel_rms(kfr_amd.convolve(a, b), SP[f"conv_{tn}"]) < tol
+    assert oracle.rel_rms(kfr_amd.correlate(a, b), SP[f"corr_{tn}"]) < tol
+    assert oracle.rel_rms(kfr_amd.autocorrelate(a), SP[f"acorr_{tn}"]) < tol
+    out = kfr_amd.convolve(SP["cconv_a"], SP["cconv_b"])
+    assert np.allclose(out.imag, 0) and oracle.rel_rms(out, SP["cconv"]) < 1e-5
+
+
+def test_convolve_filter_known_answers_and_state():
+    for dt in (np.float32, np.float64):
+        a = np.array(KAT["conv_a"], dtype=dt)
+        b = np.array(KAT["conv_b"], dtype=dt)
+        f = kfr_amd.convolve_filter(a)
+        dest = np.zeros(5, dtype=dt)
+        f.apply(dest, b)
+        assert np.sqrt(np.mean((dest - KAT["convolve_filter_first5"]) ** 2)) < 1e-4
+        f.apply(dest, b)   # the state carries over: the second block differs
+        assert np.sqrt(np.mean((dest - KAT["convolve_filter_first5"]) ** 2)) > 1e-4
+        f.reset()
+        f.apply(dest, b)
+        assert np.sqrt(np.mean((dest - KAT["convolve_filter_first5"]) ** 2)) < 1e-4
+
+
+@pytest.mark.parametrize("tn", ["f32", "f64"])
+def test_convolve_filter_streaming_matches_reference(tn):
+    dt = np.float32 if tn == "f32" else np.float64
+    taps, sig = SP[f"filt_taps_{tn}"], SP[f"filt_in_{tn}"]
+    f = kfr_amd.convolve_filter(taps, 64)
+    chunks = [sig[:1], sig[1:700], sig[700:701], sig[701:3000], sig[3000:]]
+    outs = []
+    for c in chunks:
+        o = np.empty_like(c)
+        f.apply(o, c.copy())
+        outs.append(o)
+    out = np.concatenate(outs)
+    assert oracle.rel_rms(out, SP[f"filt_out_{tn}"]) < (2e-5 if tn == "f32" else 1e-12)
+
+
+def test_convolve_filter_4095_taps_block_1024():
+    """the convolution filter of the C API with 4095 taps and block 1024 (Hann-windowed sinc)"""
+    n = 4095
+    t = np.arange(n) - (n - 1) / 2
+    h = (0.4 * np.sinc(0.4 * t) * np.hanning(n)).astype(np.float32)
+    x = oracle.gaussian(3, 20000, np.float32)
+    f = kfr_amd.convolve_filter(h, 1024)
+    y = np.empty_like(x)
+    f.apply(y[:5000], x[:5000].copy())
+    f.apply(y[5000:], x[5000:].copy())
+    ref = oracle.restated.convolve(x.astype(np.float64), h.astype(np.float64))[: x.size]
+    assert oracle.rel_rms(y, ref) < 1e-5
+
+
+def test_batched_conv_forward_and_adjoint_fd_fp64():
+    torch = pytest.importorskip("torch")
+    from kfr_amd.autograd import fft_convolve
+    torch.manual_seed(0)
+    nx, nh, B = 4096, 255, 2
+    x = torch.from_numpy(oracle.gaussian(1, B * nx, np.float64).reshape(B, nx)).cuda().requires_grad_(True)
+    h = torch.from_numpy(oracle.gaussian(2, B * nh, np.float64).reshape(B, nh)).cuda().requires_grad_(True)
+    y = fft_convolve(x, h)
+    yr = np.stack([np.convolve(x[b].detach().cpu().numpy(), h[b].detach().cpu().numpy()) for b in range(B)])
+    assert oracle.rel_rms(y.detach().cpu().numpy(), yr) < 1e-13
+    w = torch.from_numpy(oracle.gaussian(3, B * (nx + nh - 1), np.float64).reshape(B, -1)).cuda()
+    (y * w).sum().backward()
+    # finite differences at 64 random coordinates (central, FP64)
+    rng = np.random.default_rng(7)
+
+    def loss(xv, hv):
+        return float((fft_convolve(xv, hv) * w).sum())
+
+    eps = 1e-3
+    worst = 0.0
+    for _ in range(32):
+        b, i = int(rng.integers(B)), int(rng.integers(nx))
+        xp = x.detach().clone()
+        xm = x.detach().clone()
+        xp[b, i] += eps
+        xm[b, i] -= eps
+        fd = (loss(xp, h.detach()) - loss(xm, h.detach())) / (2 * eps)
+        worst = max(worst, abs(fd - float(x.grad[b, i])))
+    for _ in range(32):
+        b, j = int(rng.integers(B)), int(rng.integers(nh))
+        hp = h.detach().clone()
+        hm = h.detach().clone()
+        hp[b, j] += eps
+        hm[b, j] -= eps
+        fd = (loss(x.detach(), hp) - loss(x.detach(), hm)) / (2 * eps)
+        worst = max(worst, abs(fd - float(h.grad[b, j])))
+    print(f"adjoint vs central differences (FP64, 64 coordinates): max abs diff {worst:.3e}", flush=True)
+    assert worst < 1e-8
+
+
+def test_batched_conv_adjoint_matches_torch_fft_reference():
+    torch = pytest.importorskip("torch")
+    from kfr_amd.autograd import fft_convolve
+    nx, nh, B = 1 << 16, 4095, 4
+    x = torch.from_numpy(oracle.gaussian(11, B * nx, np.float32).reshape(B, nx)).cuda().requires_grad_(True)
+    h = torch.from_numpy(oracle.gaussian(12, nh, np.float32)).cuda().requires_grad_(True)
+    y = fft_convolve(x, h)
+    g = torch.from_numpy(oracle.gaussian(13, B * (nx + nh - 1), np.float32).reshape(B, -1)).cuda()
+    y.backward(g)
+    # torch.fft reference gradient in float64 on the CPU
+    xr = x.detach().double().cpu().requires_grad_(True)
+    hr = h.detach().double().cpu().requires_grad_(True)
+    S = 1 << 17
+    yr = torch.fft.irfft(torch.fft.rfft(xr, S) * torch.fft.rfft(hr, S), S)[:, : nx + nh - 1]
+    yr.backward(g.double().cpu())
+    assert oracle.rel_rms(y.detach().cpu().numpy(), yr.detach().numpy()) < 1e-5
+    assert oracle.rel_rms(x.grad.cpu().numpy(), xr.grad.numpy()) < 1e-5
+    assert oracle.rel_rms(h.grad.cpu().numpy(), hr.grad.numpy()) < 1e-5
