@@ -46,6 +46,22 @@ def flops(kind, n, batch):
     return f if kind == "c2c" else f / 2.0
 
 
+def in_bytes_of(c):
+    kind, dt, n, batch = c
+    es = 4 if dt == "f32" else 8
+    return (2 if kind == "c2c" else 1) * es * n * batch
+
+
+def out_size_bytes(c):
+    kind, dt, n, batch = c
+    es = 4 if dt == "f32" else 8
+    if kind == "c2c":
+        return 2 * es * n * batch
+    if kind == "r2c":
+        return 2 * es * (n // 2 + 1) * batch
+    return es * n * batch
+
+
 def alg_bytes(kind, dt, n, batch):
     """Algorithmic HBM bytes: one read of the input and one write of the output."""
     es = 4 if dt == "f32" else 8
@@ -95,8 +111,12 @@ def main() -> int:
     in_bytes = 1 << 30
     src32 = torch.randn(in_bytes // 4, generator=g, device=dev, dtype=torch.float32)
     src64 = torch.randn(in_bytes // 8, generator=g, device=dev, dtype=torch.float64)
-    out_bytes = max((1 << 30), 8 * (30720000 // 2 + 1) * 8 + 64)
-    dst = torch.empty(out_bytes // 4, device=dev, dtype=torch.float32)
+    # the output buffer holds the largest output of any configuration (R2C writes n/2+1 complex values
+    # per item, more than its input)
+    out_bytes = max(out_size_bytes(c) for c in cfgs)
+    for c in cfgs:
+        assert in_bytes_of(c) <= in_bytes, c
+    dst = torch.empty((out_bytes + 3) // 4, device=dev, dtype=torch.float32)
 
     def run_cfg(c):
         kind, dt, n, batch = c
@@ -233,20 +253,25 @@ def cpu_baseline(cfgs):
     tot_f = 0.0
     tot_t = 0.0
     sizes = []
-    for kind, dt, n, batch in cfgs:
-        if kind != "c2c" or dt != "f32":
-            continue
-        b = max(threads, min(batch, (1 << 22) // n))
-        x = oracle.complex_gaussian(11, (b, n), np.complex64)
+    c2c = [c for c in cfgs if c[0] == "c2c" and c[1] == "f32"]
+    pts = max(max(min(c[3], max(1, (1 << 21) // c[2])) * c[2] for c in c2c), 1)
+    pool = oracle.complex_gaussian(11, pts, np.complex64)
+    used = 0
+    for kind, dt, n, batch in c2c:
+        b = min(batch, max(1, (1 << 21) // n))
+        x = pool[: b * n].reshape(b, n)
         out = np.empty_like(x)
-        t = ref.time(0, 0, n, b, False, threads, 3, out, x)
+        th = min(threads, b)
+        t = ref.time(0, 0, n, b, False, th, 2, out, x)
         tot_f += flops("c2c", n, b)
         tot_t += t
+        used = max(used, th)
         sizes.append(f"{n}x{b}")
-    return {"value": round(tot_f / tot_t * 1e-9, 2), "unit": "GFLOP/s", "cores": threads, "kind": "reference",
+    return {"value": round(tot_f / tot_t * 1e-9, 2), "unit": "GFLOP/s", "cores": used, "kind": "reference",
             "arch": ref.version(),
-            "sample": "complex-f32 forward, N=2^4..2^24 with 2^22 points per size (" + ", ".join(sizes) +
-                      "), median of 3 runs per size, batch split over threads"}
+            "sample": "complex-f32 forward, N=2^4..2^24, 2^21 points per size (one transform at N>2^21): " +
+                      ", ".join(sizes) + "; median of 2 runs per size, batch split over up to "
+                      f"{threads} threads"}
 
 
 if __name__ == "__main__":
