@@ -185,7 +185,7 @@ KFR_AMD_API void kfr_filter_delete_plan_f64(KFR_FILTER_F64* plan);
 
 /* ============================== Part 2: extensions ============================== */
 
-/* Batched transforms on device pointers, enqueued on `stream` (a hipStream_t; NULL = the library's
+/* Batched transforms on device pointers, enqueued on `stream` (a hipStream_t; NULL = the default
  * stream) and not synchronised. Transforms are contiguous: item b starts at b*size (complex),
  * b*size reals / b*complex_size complex values (real), b*size reals (DCT). Returns 0 on success,
  * -1 on error (see kfr_last_error). */

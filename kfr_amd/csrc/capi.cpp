@@ -82,7 +82,8 @@ bool is_device_ptr(const void* p)
     return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged || a.type == hipMemoryTypeUnified;
 }
 
-hipStream_t as_stream(void* s) { return s ? static_cast<hipStream_t>(s) : library_stream(); }
+// NULL is the HIP default (null) stream, as in HIP and PyTorch (whose default stream handle is 0)
+hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
 // Runs fn(dev_out, dev_in) on device buffers: the user's pointers when both are device memory,
 // otherwise one staging buffer (the transforms are safe in place for a single item) with copies.

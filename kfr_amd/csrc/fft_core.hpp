@@ -446,32 +446,103 @@ struct fourstep_args
     long long n = 1;
 };
 
-// contiguous load of the block's transforms into LDS (element fastest)
+// Load and store phases. Each thread stages its (at most pmax) elements in registers, so that all of
+// its global loads (or stores) are in flight together; the host guarantees tpb*n <= pmax*blockDim.
+
+template <typename T>
+__device__ __forceinline__ bool plain_contiguous(const xload<T>& ld, long long n)
+{
+    return ld.mode == ld_plain && ld.pdist == n && ld.mtab == nullptr && ld.lim == LLONG_MAX;
+}
+template <typename T>
+__device__ __forceinline__ bool plain_contiguous(const xstore<T>& st, long long n)
+{
+    return st.mode == st_plain && st.pdist == n && st.mtab == nullptr && st.lim == LLONG_MAX;
+}
+
+// the block's transforms, element index fastest (e = tb*n + i); LDS layout lds[e] (ld = n, ls = 1)
 template <typename T>
 __device__ __forceinline__ void block_load(const lds_geom& G, cx<T>* lds, long long first, long long batch,
                                            const xload<T>& ld, const fastdiv& ndiv)
 {
-    const int total = G.tpb * G.n;
-    for (int e = threadIdx.x; e < total; e += blockDim.x)
+    const int total     = G.tpb * G.n;
+    const int bs        = blockDim.x;
+    const int tid       = threadIdx.x;
+    const long long rem = (batch - first) * (long long)G.n;
+    const int lim       = rem < total ? (int)rem : total;
+    cx<T> v[pmax];
+    if (plain_contiguous(ld, G.n))
     {
-        const int tb      = (int)fdiv((uint32_t)e, ndiv);
-        const int i       = e - tb * G.n;
-        const long long g = first + tb;
-        lds[(size_t)tb * G.ld + (size_t)i * G.ls] = g < batch ? ld.load(g, i) : cx<T>{ T(0), T(0) };
+        const cx<T>* src = static_cast<const cx<T>*>(ld.p) + (first + ld.b0) * (long long)G.n;
+#pragma unroll
+        for (int k = 0; k < pmax; ++k)
+        {
+            const int e = tid + k * bs;
+            v[k]        = e < lim ? src[e] : cx<T>{ T(0), T(0) };
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int k = 0; k < pmax; ++k)
+        {
+            const int e = tid + k * bs;
+            v[k]        = { T(0), T(0) };
+            if (e < lim)
+            {
+                const int tb = (int)fdiv((uint32_t)e, ndiv);
+                v[k]         = ld.load(first + tb, e - tb * G.n);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < pmax; ++k)
+    {
+        const int e = tid + k * bs;
+        if (e < total)
+            lds[e] = v[k];
     }
 }
 template <typename T>
 __device__ __forceinline__ void block_store(const lds_geom& G, const cx<T>* lds, long long first, long long batch,
                                             const xstore<T>& st, const fastdiv& ndiv)
 {
-    const int total = G.tpb * G.n;
-    for (int e = threadIdx.x; e < total; e += blockDim.x)
+    const int total     = G.tpb * G.n;
+    const int bs        = blockDim.x;
+    const int tid       = threadIdx.x;
+    const long long rem = (batch - first) * (long long)G.n;
+    const int lim       = rem < total ? (int)rem : total;
+    cx<T> v[pmax];
+#pragma unroll
+    for (int k = 0; k < pmax; ++k)
     {
-        const int tb      = (int)fdiv((uint32_t)e, ndiv);
-        const int i       = e - tb * G.n;
-        const long long g = first + tb;
-        if (g < batch)
-            st.store(g, i, lds[(size_t)tb * G.ld + (size_t)i * G.ls]);
+        const int e = tid + k * bs;
+        if (e < total)
+            v[k] = lds[e];
+    }
+    if (plain_contiguous(st, G.n))
+    {
+        cx<T>* dst = static_cast<cx<T>*>(st.p) + (first + st.b0) * (long long)G.n;
+#pragma unroll
+        for (int k = 0; k < pmax; ++k)
+        {
+            const int e = tid + k * bs;
+            if (e < lim)
+                dst[e] = v[k];
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int k = 0; k < pmax; ++k)
+        {
+            const int e = tid + k * bs;
+            if (e < lim)
+            {
+                const int tb = (int)fdiv((uint32_t)e, ndiv);
+                st.store(first + tb, e - tb * G.n, v[k]);
+            }
+        }
     }
 }
 
@@ -483,26 +554,59 @@ __global__ void __launch_bounds__(1024) k_fft(lds_geom G, const cx<T>* __restric
     cx<T>* lds            = reinterpret_cast<cx<T>*>(smem_raw);
     const long long first = (long long)blockIdx.x * G.tpb;
     const fft_ctx<T> c    = make_ctx<T>(G, lds, first, batch);
+    cx<T> v[pmax];
     if constexpr (ROLE == role_direct)
     {
         block_load(G, lds, first, batch, ld, fs.ndiv);
     }
     else if constexpr (ROLE == role_col)
     {
+        // column col of batch item bl: elements col + n1*i; consecutive lanes take consecutive columns
         const uint32_t bl = fdiv((uint32_t)c.g, fs.n1div);
         const int col     = (int)((uint32_t)c.g - bl * (uint32_t)fs.n1);
-        for (int i = c.t; i < G.n; i += G.tpt)
-            c.L[i * G.ls] = c.valid ? ld.load(bl, (long long)col + (long long)fs.n1 * i) : cx<T>{ T(0), T(0) };
+        if (ld.mode == ld_plain && ld.mtab == nullptr && ld.lim == LLONG_MAX)
+        {
+            const cx<T>* src = static_cast<const cx<T>*>(ld.p) + ((long long)bl + ld.b0) * ld.pdist + col;
+#pragma unroll
+            for (int k = 0; k < pmax; ++k)
+            {
+                const int i = c.t + k * G.tpt;
+                v[k]        = (i < G.n && c.valid) ? src[(long long)fs.n1 * i] : cx<T>{ T(0), T(0) };
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int k = 0; k < pmax; ++k)
+            {
+                const int i = c.t + k * G.tpt;
+                v[k] = (i < G.n && c.valid) ? ld.load(bl, (long long)col + (long long)fs.n1 * i) : cx<T>{ T(0), T(0) };
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < pmax; ++k)
+        {
+            const int i = c.t + k * G.tpt;
+            if (i < G.n)
+                c.L[i * G.ls] = v[k];
+        }
     }
     else
     {
-        const int total = G.tpb * G.n;
-        for (int e = threadIdx.x; e < total; e += blockDim.x)
+        // row g of the intermediate, contiguous
+        const cx<T>* src = fs.scratch + c.g * (long long)G.n;
+#pragma unroll
+        for (int k = 0; k < pmax; ++k)
         {
-            const int tb      = (int)fdiv((uint32_t)e, fs.ndiv);
-            const int i       = e - tb * G.n;
-            const long long g = first + tb;
-            lds[(size_t)tb * G.ld + i] = g < batch ? fs.scratch[g * G.n + i] : cx<T>{ T(0), T(0) };
+            const int i = c.t + k * G.tpt;
+            v[k]        = (i < G.n && c.valid) ? src[i] : cx<T>{ T(0), T(0) };
+        }
+#pragma unroll
+        for (int k = 0; k < pmax; ++k)
+        {
+            const int i = c.t + k * G.tpt;
+            if (i < G.n)
+                c.L[i * G.ls] = v[k];
         }
     }
     __syncthreads();
@@ -519,28 +623,65 @@ __global__ void __launch_bounds__(1024) k_fft(lds_geom G, const cx<T>* __restric
             const int col     = (int)((uint32_t)c.g - bl * (uint32_t)fs.n1);
             cx<T>* dst        = fs.scratch + (long long)bl * fs.n + col;
             const unsigned long long mask = (1ull << fs.lbits) - 1;
-            for (int i = c.t; i < G.n; i += G.tpt)
+#pragma unroll
+            for (int k = 0; k < pmax; ++k)
             {
-                const unsigned long long m = (unsigned long long)col * (unsigned long long)i; // < n
-                const cx<T> w              = cmul(fs.ta[m >> fs.lbits], fs.tb[m & mask]);
-                dst[(long long)fs.n1 * i]  = ctw<INV>(c.L[i * G.ls], w);
+                const int i = c.t + k * G.tpt;
+                if (i < G.n)
+                    v[k] = c.L[i * G.ls];
+            }
+#pragma unroll
+            for (int k = 0; k < pmax; ++k)
+            {
+                const int i = c.t + k * G.tpt;
+                if (i < G.n)
+                {
+                    const unsigned long long m = (unsigned long long)col * (unsigned long long)i; // < n
+                    const cx<T> w              = cmul(fs.ta[m >> fs.lbits], fs.tb[m & mask]);
+                    dst[(long long)fs.n1 * i]  = ctw<INV>(v[k], w);
+                }
             }
         }
     }
     else
     {
-        // transform index fastest: consecutive lanes write consecutive output columns
-        const int total = G.tpb * G.n;
-        for (int e = threadIdx.x; e < total; e += blockDim.x)
+        // transposed store: transform index fastest, so consecutive lanes write consecutive columns k2.
+        // blockDim = tpt*tpb, so a thread keeps one transform and steps i by tpt.
+        const int tid    = threadIdx.x;
+        const int tb     = tid % G.tpb;
+        const int i0     = tid / G.tpb;
+        const long long g = first + tb;
+        if (g < batch)
         {
-            const int i       = e / G.tpb;
-            const int tb      = e - i * G.tpb;
-            const long long g = first + tb;
-            if (g < batch)
+            const uint32_t bl = fdiv((uint32_t)g, fs.n2div);
+            const int k2      = (int)((uint32_t)g - bl * (uint32_t)fs.n2);
+#pragma unroll
+            for (int k = 0; k < pmax; ++k)
             {
-                const uint32_t bl = fdiv((uint32_t)g, fs.n2div);
-                const int k2      = (int)((uint32_t)g - bl * (uint32_t)fs.n2);
-                st.store(bl, (long long)k2 + (long long)fs.n2 * i, lds[(size_t)tb * G.ld + i]);
+                const int i = i0 + k * G.tpt;
+                if (i < G.n)
+                    v[k] = lds[(size_t)tb * G.ld + i];
+            }
+            if (st.mode == st_plain && st.mtab == nullptr && st.lim == LLONG_MAX)
+            {
+                cx<T>* dst = static_cast<cx<T>*>(st.p) + ((long long)bl + st.b0) * st.pdist + k2;
+#pragma unroll
+                for (int k = 0; k < pmax; ++k)
+                {
+                    const int i = i0 + k * G.tpt;
+                    if (i < G.n)
+                        dst[(long long)fs.n2 * i] = v[k];
+                }
+            }
+            else
+            {
+#pragma unroll
+                for (int k = 0; k < pmax; ++k)
+                {
+                    const int i = i0 + k * G.tpt;
+                    if (i < G.n)
+                        st.store(bl, (long long)k2 + (long long)fs.n2 * i, v[k]);
+                }
             }
         }
     }

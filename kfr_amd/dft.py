@@ -8,6 +8,7 @@ GPU and back) or torch tensors on the GPU (used in place). No scaling is applied
 from __future__ import annotations
 
 import ctypes as C
+import sys
 import threading
 
 import numpy as np
@@ -47,9 +48,15 @@ def _tag(dtype) -> str:
 
 
 def _stream_ptr(stream):
+    """A hipStream_t handle: the given torch stream or handle, else torch's current stream when torch
+    is in use, else the default stream (NULL)."""
     if stream is None:
+        torch = sys.modules.get("torch")
+        if torch is not None and torch.cuda.is_available():
+            return torch.cuda.current_stream().cuda_stream or None
         return None
-    return getattr(stream, "cuda_stream", stream)
+    h = getattr(stream, "cuda_stream", stream)
+    return h or None
 
 
 class dft_plan:
