@@ -500,7 +500,7 @@ __device__ __forceinline__ void block_load(const lds_geom& G, cx<T>* lds, long l
     {
         const int e = tid + k * bs;
         if (e < total)
-            lds[e] = v[k];
+            lds[lds_pad(e)] = v[k];
     }
 }
 template <typename T>
@@ -518,7 +518,7 @@ __device__ __forceinline__ void block_store(const lds_geom& G, const cx<T>* lds,
     {
         const int e = tid + k * bs;
         if (e < total)
-            v[k] = lds[e];
+            v[k] = lds[lds_pad(e)];
     }
     if (plain_contiguous(st, G.n))
     {
@@ -588,7 +588,7 @@ __global__ void __launch_bounds__(1024) k_fft(lds_geom G, const cx<T>* __restric
         {
             const int i = c.t + k * G.tpt;
             if (i < G.n)
-                c.L[i * G.ls] = v[k];
+                c.at(i) = v[k];
         }
     }
     else
@@ -606,7 +606,7 @@ __global__ void __launch_bounds__(1024) k_fft(lds_geom G, const cx<T>* __restric
         {
             const int i = c.t + k * G.tpt;
             if (i < G.n)
-                c.L[i * G.ls] = v[k];
+                c.at(i) = v[k];
         }
     }
     __syncthreads();
@@ -628,7 +628,7 @@ __global__ void __launch_bounds__(1024) k_fft(lds_geom G, const cx<T>* __restric
             {
                 const int i = c.t + k * G.tpt;
                 if (i < G.n)
-                    v[k] = c.L[i * G.ls];
+                    v[k] = c.at(i);
             }
 #pragma unroll
             for (int k = 0; k < pmax; ++k)
@@ -660,7 +660,7 @@ __global__ void __launch_bounds__(1024) k_fft(lds_geom G, const cx<T>* __restric
             {
                 const int i = i0 + k * G.tpt;
                 if (i < G.n)
-                    v[k] = lds[(size_t)tb * G.ld + i];
+                    v[k] = lds[lds_pad(tb * G.ld + i)];
             }
             if (st.mode == st_plain && st.mtab == nullptr && st.lim == LLONG_MAX)
             {
@@ -724,7 +724,7 @@ struct core_fft
             return 1;
         int t = std::max(1, 256 / s.tpt);
         // at most 64 KiB of LDS per block so that two blocks share a CU
-        const size_t per = (size_t)s.n * sizeof(cx<T>);
+        const size_t per = lds_padded_elems((size_t)s.n) * sizeof(cx<T>);
         t                = (int)std::max<size_t>(1, std::min<size_t>(t, 65536 / per));
         return t;
     }
@@ -831,7 +831,7 @@ struct core_fft
         if (!fourstep)
         {
             lds_geom G      = make_geom(S, tpb, 0, 1, (int)n);
-            const size_t sh = (size_t)tpb * n * sizeof(cx<T>);
+            const size_t sh = lds_padded_elems((size_t)tpb * n) * sizeof(cx<T>);
             fourstep_args<T> fs;
             fs.ndiv = make_fastdiv((uint32_t)n);
             allow_lds(k_fft<T, INV, role_direct>, sh);
@@ -843,8 +843,8 @@ struct core_fft
         scratch.ensure((size_t)std::min<long long>(chunk, batch) * n * sizeof(cx<T>));
         lds_geom GA      = make_geom(SA, C, 1, C, 1);
         lds_geom GB      = make_geom(SB, R, 0, 1, (int)n1 + 1);
-        const size_t shA = (size_t)C * n2 * sizeof(cx<T>);
-        const size_t shB = (size_t)R * (n1 + 1) * sizeof(cx<T>);
+        const size_t shA = lds_padded_elems((size_t)C * n2) * sizeof(cx<T>);
+        const size_t shB = lds_padded_elems((size_t)R * (n1 + 1)) * sizeof(cx<T>);
         fourstep_args<T> fs;
         fs.scratch = scratch.as<cx<T>>();
         fs.ta      = t + twTA;

@@ -340,14 +340,24 @@ struct lds_geom
     pass_desc pass[max_passes];
 };
 
+// LDS layout: logical element x lives at x + x/16, one padding slot per 16 elements, so that the
+// power-of-two strides of the Stockham passes (stride-R writes of the first pass, one transform per
+// lane for short lengths) spread over the banks instead of hitting one.
+__device__ __forceinline__ int lds_pad(int x) { return x + (x >> 4); }
+__host__ __device__ inline size_t lds_padded_elems(size_t logical) { return logical + logical / 16 + 1; }
+
 template <typename T>
 struct fft_ctx
 {
-    cx<T>* L;    // this transform's LDS base
+    cx<T>* lds;  // the block's LDS
+    int base;    // this transform's logical LDS offset (tb * ld)
+    int ls;      // logical element stride
     int t;       // thread index within the transform
     int tb;      // transform index within the block
     bool valid;  // transform exists (batch tail)
     long long g; // global transform index
+    // element i of this transform (padded layout, see lds_pad)
+    __device__ __forceinline__ cx<T>& at(int i) const { return lds[lds_pad(base + i * ls)]; }
 };
 
 // one radix pass, LDS -> LDS
@@ -357,7 +367,6 @@ __device__ __forceinline__ void radix_pass(const lds_geom& G, const pass_desc& P
 {
     constexpr int M = pmax / R > 0 ? pmax / R : 1;
     const int nb    = G.n / R;
-    const int ls    = G.ls;
     cx<T> v[M][R];
     int kk[M];
 #pragma unroll
@@ -368,7 +377,7 @@ __device__ __forceinline__ void radix_pass(const lds_geom& G, const pass_desc& P
         {
 #pragma unroll
             for (int r = 0; r < R; ++r)
-                v[m][r] = c.L[(j + r * nb) * ls];
+                v[m][r] = c.at(j + r * nb);
         }
     }
 #pragma unroll
@@ -400,7 +409,7 @@ __device__ __forceinline__ void radix_pass(const lds_geom& G, const pass_desc& P
             const int base = (j - kk[m]) * R + kk[m];
 #pragma unroll
             for (int r = 0; r < R; ++r)
-                c.L[(base + r * P.ns) * ls] = v[m][r];
+                c.at(base + r * P.ns) = v[m][r];
         }
     }
     __syncthreads();
@@ -415,7 +424,6 @@ __device__ __forceinline__ void generic_pass(const lds_geom& G, const pass_desc&
 {
     const int R  = P.radix;
     const int nb = G.n / R;
-    const int ls = G.ls;
     const int M  = P.ns * R; // table length
     const cx<T>* w = tw + P.tw;
     cx<T> out[pmax];
@@ -436,7 +444,7 @@ __device__ __forceinline__ void generic_pass(const lds_geom& G, const pass_desc&
             cx<T> acc      = { T(0), T(0) };
             for (int r = 0; r < R; ++r)
             {
-                acc = cadd(acc, ctw<INV>(c.L[(j + r * nb) * ls], w[idx]));
+                acc = cadd(acc, ctw<INV>(c.at(j + r * nb), w[idx]));
                 idx += step;
                 if (idx >= M)
                     idx -= M;
@@ -450,7 +458,7 @@ __device__ __forceinline__ void generic_pass(const lds_geom& G, const pass_desc&
     {
         const int o = c.t + m * G.tpt;
         if (o < G.n)
-            c.L[o * ls] = out[m];
+            c.at(o) = out[m];
     }
     __syncthreads();
 }
@@ -509,7 +517,9 @@ __device__ __forceinline__ fft_ctx<T> make_ctx(const lds_geom& G, cx<T>* lds, lo
     }
     c.g     = first_g + c.tb;
     c.valid = c.g < batch;
-    c.L     = lds + (size_t)c.tb * G.ld;
+    c.lds   = lds;
+    c.base  = c.tb * G.ld;
+    c.ls    = G.ls;
     return c;
 }
 

@@ -96,7 +96,7 @@ __global__ void __launch_bounds__(1024) k_bluestein(lds_geom G, const cx<T>* __r
     __syncthreads();
     fft_passes<T, false>(G, tw, c);
     for (int i = c.t; i < G.n; i += G.tpt)
-        c.L[i * G.ls] = INV ? cmul(c.L[i * G.ls], H[i]) : cmulc(c.L[i * G.ls], H[i]);
+        c.at(i) = INV ? cmul(c.at(i), H[i]) : cmulc(c.at(i), H[i]);
     __syncthreads();
     fft_passes<T, true>(G, tw, c);
     block_store(G, lds, first, batch, st, mdiv); // st carries the chirp and the truncation
@@ -241,7 +241,7 @@ struct gpu_dft<T>::impl
         if (algo == BS_FUSED)
         {
             lds_geom G      = make_geom(bsS, bs_tpb, 0, 1, (int)M);
-            const size_t sh = (size_t)bs_tpb * M * sizeof(cx<T>);
+            const size_t sh = lds_padded_elems((size_t)bs_tpb * M) * sizeof(cx<T>);
             allow_lds(k_bluestein<T, INV>, sh);
             hipLaunchKernelGGL((k_bluestein<T, INV>), dim3(nblocks(batch, bs_tpb)), dim3(bsS.tpt * bs_tpb), sh, s, G,
                                bs_tw.as<cx<T>>(), batch, H.as<cx<T>>(), lc, sc, make_fastdiv((uint32_t)M));
@@ -379,14 +379,14 @@ __global__ void __launch_bounds__(1024) k_real_fwd(lds_geom G, const cx<T>* __re
     for (long long k = c.t; k <= count; k += G.tpt)
     {
         if (k == 0)
-            repack_dc(o, csize, fmt, c.L[0]);
+            repack_dc(o, csize, fmt, c.at(0));
         else if (k == count)
         {
             if (csize % 2 == 0)
-                o[csize / 2] = cconj(c.L[(csize / 2) * G.ls]);
+                o[csize / 2] = cconj(c.at((int)(csize / 2)));
         }
         else
-            repack_pair(o, rtw, csize, k, c.L[k * G.ls], c.L[(csize - k) * G.ls]);
+            repack_pair(o, rtw, csize, k, c.at((int)k), c.at((int)(csize - k)));
     }
 }
 
@@ -425,7 +425,7 @@ struct gpu_dft_real<T>::impl
         {
             core_fft<T>& f  = *c->core;
             lds_geom G      = make_geom(f.S, f.tpb, 0, 1, (int)csize);
-            const size_t sh = (size_t)f.tpb * csize * sizeof(cx<T>);
+            const size_t sh = lds_padded_elems((size_t)f.tpb * csize) * sizeof(cx<T>);
             allow_lds(k_real_fwd<T>, sh);
             hipLaunchKernelGGL((k_real_fwd<T>), dim3(nblocks(batch, f.tpb)), dim3(f.S.tpt * f.tpb), sh, s, G,
                                f.tw.template as<cx<T>>(), batch, ld, out, odist, rtw.template as<cx<T>>(), fmt,
