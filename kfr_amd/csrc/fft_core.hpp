@@ -312,9 +312,10 @@ struct stockham
 
 inline bool lds_radix(int r) { return r == 2 || r == 3 || r == 4 || r == 5 || r == 7 || r == 8 || r == 16; }
 
-// Factor n into GPU passes. Power-of-2 part: radix 16 first, remainder 8/4/2; then 7, 5, 3; any
-// other prime factor up to 127 becomes a direct DFT pass, placed last.
-inline bool make_stockham(long long nn, stockham& S)
+// Factor n into GPU passes. Power-of-2 part: the largest butterfly that fits a thread's pm values
+// (16 for f32, 8 for f64) first, remainder 8/4/2; then 7, 5, 3; any other prime factor up to 127
+// becomes a direct DFT pass, placed last.
+inline bool make_stockham(long long nn, stockham& S, int pm)
 {
     S = stockham{};
     if (nn < 1 || nn > (1 << 28))
@@ -329,10 +330,11 @@ inline bool make_stockham(long long nn, stockham& S)
         m /= 2;
         ++a;
     }
-    while (a >= 4)
+    const int rbits = pm >= 16 ? 4 : 3;
+    while (a >= rbits)
     {
-        pw2.push_back(16);
-        a -= 4;
+        pw2.push_back(1 << rbits);
+        a -= rbits;
     }
     if (a == 3)
         pw2.push_back(8);
@@ -373,13 +375,13 @@ inline bool make_stockham(long long nn, stockham& S)
         if (g)
         {
             off += ns * r;
-            tpt = std::max(tpt, (n + pmax - 1) / pmax);
+            tpt = std::max(tpt, (n + pm - 1) / pm);
         }
         else
         {
             if (ns > 1)
                 off += ns * (r - 1);
-            const int M  = std::max(1, pmax / r);
+            const int M  = std::max(1, pm / r);
             const int nb = n / r;
             tpt          = std::max(tpt, (nb + M - 1) / M);
         }
@@ -414,6 +416,12 @@ inline lds_geom make_geom(const stockham& S, int tpb, int mapping, int ls, int l
     G.mapping = mapping;
     G.ls      = ls;
     G.ld      = ld;
+    static const int padshift = []
+    {
+        const char* e = std::getenv("KFR_AMD_LDS_PADSHIFT");
+        return e ? std::max(1, std::min(30, atoi(e))) : 4;
+    }();
+    G.padshift = padshift;
     for (int p = 0; p < G.np; ++p)
     {
         G.pass[p].radix  = S.radix[p];
@@ -470,12 +478,12 @@ __device__ __forceinline__ void block_load(const lds_geom& G, cx<T>* lds, long l
     const int tid       = threadIdx.x;
     const long long rem = (batch - first) * (long long)G.n;
     const int lim       = rem < total ? (int)rem : total;
-    cx<T> v[pmax];
+    cx<T> v[pmax_of<T>];
     if (plain_contiguous(ld, G.n))
     {
         const cx<T>* src = static_cast<const cx<T>*>(ld.p) + (first + ld.b0) * (long long)G.n;
 #pragma unroll
-        for (int k = 0; k < pmax; ++k)
+        for (int k = 0; k < pmax_of<T>; ++k)
         {
             const int e = tid + k * bs;
             v[k]        = e < lim ? src[e] : cx<T>{ T(0), T(0) };
@@ -484,7 +492,7 @@ __device__ __forceinline__ void block_load(const lds_geom& G, cx<T>* lds, long l
     else
     {
 #pragma unroll
-        for (int k = 0; k < pmax; ++k)
+        for (int k = 0; k < pmax_of<T>; ++k)
         {
             const int e = tid + k * bs;
             v[k]        = { T(0), T(0) };
@@ -496,11 +504,11 @@ __device__ __forceinline__ void block_load(const lds_geom& G, cx<T>* lds, long l
         }
     }
 #pragma unroll
-    for (int k = 0; k < pmax; ++k)
+    for (int k = 0; k < pmax_of<T>; ++k)
     {
         const int e = tid + k * bs;
         if (e < total)
-            lds[lds_pad(e)] = v[k];
+            lds[lds_pad(e, G.padshift)] = v[k];
     }
 }
 template <typename T>
@@ -512,19 +520,19 @@ __device__ __forceinline__ void block_store(const lds_geom& G, const cx<T>* lds,
     const int tid       = threadIdx.x;
     const long long rem = (batch - first) * (long long)G.n;
     const int lim       = rem < total ? (int)rem : total;
-    cx<T> v[pmax];
+    cx<T> v[pmax_of<T>];
 #pragma unroll
-    for (int k = 0; k < pmax; ++k)
+    for (int k = 0; k < pmax_of<T>; ++k)
     {
         const int e = tid + k * bs;
         if (e < total)
-            v[k] = lds[lds_pad(e)];
+            v[k] = lds[lds_pad(e, G.padshift)];
     }
     if (plain_contiguous(st, G.n))
     {
         cx<T>* dst = static_cast<cx<T>*>(st.p) + (first + st.b0) * (long long)G.n;
 #pragma unroll
-        for (int k = 0; k < pmax; ++k)
+        for (int k = 0; k < pmax_of<T>; ++k)
         {
             const int e = tid + k * bs;
             if (e < lim)
@@ -534,7 +542,7 @@ __device__ __forceinline__ void block_store(const lds_geom& G, const cx<T>* lds,
     else
     {
 #pragma unroll
-        for (int k = 0; k < pmax; ++k)
+        for (int k = 0; k < pmax_of<T>; ++k)
         {
             const int e = tid + k * bs;
             if (e < lim)
@@ -546,15 +554,15 @@ __device__ __forceinline__ void block_store(const lds_geom& G, const cx<T>* lds,
     }
 }
 
-template <typename T, bool INV, int ROLE>
-__global__ void __launch_bounds__(1024) k_fft(lds_geom G, const cx<T>* __restrict__ tw, long long batch,
+template <typename T, bool INV, int ROLE, bool MIXED>
+__global__ void __launch_bounds__(max_block_of(MIXED)) k_fft(lds_geom G, const cx<T>* __restrict__ tw, long long batch,
                                               xload<T> ld, xstore<T> st, fourstep_args<T> fs)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lds            = reinterpret_cast<cx<T>*>(smem_raw);
     const long long first = (long long)blockIdx.x * G.tpb;
     const fft_ctx<T> c    = make_ctx<T>(G, lds, first, batch);
-    cx<T> v[pmax];
+    cx<T> v[pmax_of<T>];
     if constexpr (ROLE == role_direct)
     {
         block_load(G, lds, first, batch, ld, fs.ndiv);
@@ -568,7 +576,7 @@ __global__ void __launch_bounds__(1024) k_fft(lds_geom G, const cx<T>* __restric
         {
             const cx<T>* src = static_cast<const cx<T>*>(ld.p) + ((long long)bl + ld.b0) * ld.pdist + col;
 #pragma unroll
-            for (int k = 0; k < pmax; ++k)
+            for (int k = 0; k < pmax_of<T>; ++k)
             {
                 const int i = c.t + k * G.tpt;
                 v[k]        = (i < G.n && c.valid) ? src[(long long)fs.n1 * i] : cx<T>{ T(0), T(0) };
@@ -577,14 +585,14 @@ __global__ void __launch_bounds__(1024) k_fft(lds_geom G, const cx<T>* __restric
         else
         {
 #pragma unroll
-            for (int k = 0; k < pmax; ++k)
+            for (int k = 0; k < pmax_of<T>; ++k)
             {
                 const int i = c.t + k * G.tpt;
                 v[k] = (i < G.n && c.valid) ? ld.load(bl, (long long)col + (long long)fs.n1 * i) : cx<T>{ T(0), T(0) };
             }
         }
 #pragma unroll
-        for (int k = 0; k < pmax; ++k)
+        for (int k = 0; k < pmax_of<T>; ++k)
         {
             const int i = c.t + k * G.tpt;
             if (i < G.n)
@@ -596,13 +604,13 @@ __global__ void __launch_bounds__(1024) k_fft(lds_geom G, const cx<T>* __restric
         // row g of the intermediate, contiguous
         const cx<T>* src = fs.scratch + c.g * (long long)G.n;
 #pragma unroll
-        for (int k = 0; k < pmax; ++k)
+        for (int k = 0; k < pmax_of<T>; ++k)
         {
             const int i = c.t + k * G.tpt;
             v[k]        = (i < G.n && c.valid) ? src[i] : cx<T>{ T(0), T(0) };
         }
 #pragma unroll
-        for (int k = 0; k < pmax; ++k)
+        for (int k = 0; k < pmax_of<T>; ++k)
         {
             const int i = c.t + k * G.tpt;
             if (i < G.n)
@@ -610,7 +618,7 @@ __global__ void __launch_bounds__(1024) k_fft(lds_geom G, const cx<T>* __restric
         }
     }
     __syncthreads();
-    fft_passes<T, INV>(G, tw, c);
+    fft_passes<T, INV, MIXED>(G, tw, c);
     if constexpr (ROLE == role_direct)
     {
         block_store(G, lds, first, batch, st, fs.ndiv);
@@ -624,14 +632,14 @@ __global__ void __launch_bounds__(1024) k_fft(lds_geom G, const cx<T>* __restric
             cx<T>* dst        = fs.scratch + (long long)bl * fs.n + col;
             const unsigned long long mask = (1ull << fs.lbits) - 1;
 #pragma unroll
-            for (int k = 0; k < pmax; ++k)
+            for (int k = 0; k < pmax_of<T>; ++k)
             {
                 const int i = c.t + k * G.tpt;
                 if (i < G.n)
                     v[k] = c.at(i);
             }
 #pragma unroll
-            for (int k = 0; k < pmax; ++k)
+            for (int k = 0; k < pmax_of<T>; ++k)
             {
                 const int i = c.t + k * G.tpt;
                 if (i < G.n)
@@ -656,17 +664,17 @@ __global__ void __launch_bounds__(1024) k_fft(lds_geom G, const cx<T>* __restric
             const uint32_t bl = fdiv((uint32_t)g, fs.n2div);
             const int k2      = (int)((uint32_t)g - bl * (uint32_t)fs.n2);
 #pragma unroll
-            for (int k = 0; k < pmax; ++k)
+            for (int k = 0; k < pmax_of<T>; ++k)
             {
                 const int i = i0 + k * G.tpt;
                 if (i < G.n)
-                    v[k] = lds[lds_pad(tb * G.ld + i)];
+                    v[k] = lds[lds_pad(tb * G.ld + i, G.padshift)];
             }
             if (st.mode == st_plain && st.mtab == nullptr && st.lim == LLONG_MAX)
             {
                 cx<T>* dst = static_cast<cx<T>*>(st.p) + ((long long)bl + st.b0) * st.pdist + k2;
 #pragma unroll
-                for (int k = 0; k < pmax; ++k)
+                for (int k = 0; k < pmax_of<T>; ++k)
                 {
                     const int i = i0 + k * G.tpt;
                     if (i < G.n)
@@ -676,7 +684,7 @@ __global__ void __launch_bounds__(1024) k_fft(lds_geom G, const cx<T>* __restric
             else
             {
 #pragma unroll
-                for (int k = 0; k < pmax; ++k)
+                for (int k = 0; k < pmax_of<T>; ++k)
                 {
                     const int i = i0 + k * G.tpt;
                     if (i < G.n)
@@ -691,6 +699,33 @@ __global__ void __launch_bounds__(1024) k_fft(lds_geom G, const cx<T>* __restric
 // A transform computed without Bluestein: one LDS kernel, or four-step in two kernels.
 
 inline size_t lds_budget() { return 128 * 1024; }
+
+inline bool is_mixed(const stockham& s)
+{
+    for (int r : s.radix)
+        if (r & (r - 1))
+            return true;
+    return false;
+}
+
+// launches the power-of-two or the mixed-radix build of a kernel role
+template <typename T, bool INV, int ROLE>
+void launch_fft(bool mixed, const lds_geom& G, const cx<T>* tw, long long batch, const xload<T>& ld,
+                const xstore<T>& st, const fourstep_args<T>& fs, unsigned grid, unsigned block, size_t sh,
+                hipStream_t s)
+{
+    if (mixed)
+    {
+        allow_lds(k_fft<T, INV, ROLE, true>, sh);
+        hipLaunchKernelGGL((k_fft<T, INV, ROLE, true>), dim3(grid), dim3(block), sh, s, G, tw, batch, ld, st, fs);
+    }
+    else
+    {
+        allow_lds(k_fft<T, INV, ROLE, false>, sh);
+        hipLaunchKernelGGL((k_fft<T, INV, ROLE, false>), dim3(grid), dim3(block), sh, s, G, tw, batch, ld, st, fs);
+    }
+    hip_check(hipGetLastError(), "fft launch");
+}
 
 template <typename T>
 struct core_fft
@@ -713,9 +748,9 @@ struct core_fft
     static bool fits_lds(size_t len)
     {
         stockham s;
-        if (!make_stockham((long long)len, s))
+        if (!make_stockham((long long)len, s, pmax_of<T>))
             return false;
-        return s.tpt <= 1024 && len * sizeof(cx<T>) <= lds_budget();
+        return s.tpt <= max_block_of(is_mixed(s)) && len * sizeof(cx<T>) <= lds_budget();
     }
 
     static int pick_tpb(const stockham& s)
@@ -734,7 +769,7 @@ struct core_fft
         n = len;
         if (fits_lds(n))
         {
-            make_stockham((long long)n, S);
+            make_stockham((long long)n, S, pmax_of<T>);
             tpb = pick_tpb(S);
             tw.reset((size_t)std::max<long long>(1, S.tw_count) * sizeof(cx<T>));
             gen_stockham_twiddles(S, tw.as<cx<T>>(), s);
@@ -787,14 +822,14 @@ struct core_fft
             if (!fits_lds(d) || !fits_lds(a))
                 continue;
             stockham sa, sb;
-            make_stockham((long long)d, sa);
-            make_stockham((long long)a, sb);
+            make_stockham((long long)d, sa, pmax_of<T>);
+            make_stockham((long long)a, sb, pmax_of<T>);
             int C = 16;
-            while (C > 1 && ((size_t)C * d * sizeof(cx<T>) > lds_budget() || C * sa.tpt > 1024 || a % C != 0))
+            const int bA = max_block_of(is_mixed(sa)), bB = max_block_of(is_mixed(sb));
+            while (C > 1 && ((size_t)C * d * sizeof(cx<T>) > lds_budget() || C * sa.tpt > bA || a % C != 0))
                 C /= 2;
             int R = 16;
-            while (R > 1 &&
-                   ((size_t)R * (a + 1) * sizeof(cx<T>) > lds_budget() || R * sb.tpt > 1024 || d % R != 0))
+            while (R > 1 && ((size_t)R * (a + 1) * sizeof(cx<T>) > lds_budget() || R * sb.tpt > bB || d % R != 0))
                 R /= 2;
             const int w      = std::min(C, R);
             const double bal = std::fabs(std::log((double)a) - std::log((double)d));
@@ -818,8 +853,8 @@ struct core_fft
             C = std::max(1, std::min(C, atoi(e)));
         if (const char* e = std::getenv("KFR_AMD_FOURSTEP_R"))
             R = std::max(1, std::min(R, atoi(e)));
-        make_stockham((long long)n2, SA);
-        make_stockham((long long)n1, SB);
+        make_stockham((long long)n2, SA, pmax_of<T>);
+        make_stockham((long long)n1, SB, pmax_of<T>);
     }
 
     template <bool INV>
@@ -834,10 +869,8 @@ struct core_fft
             const size_t sh = lds_padded_elems((size_t)tpb * n) * sizeof(cx<T>);
             fourstep_args<T> fs;
             fs.ndiv = make_fastdiv((uint32_t)n);
-            allow_lds(k_fft<T, INV, role_direct>, sh);
-            hipLaunchKernelGGL((k_fft<T, INV, role_direct>), dim3(nblocks(batch, tpb)), dim3(S.tpt * tpb), sh, s, G, t,
-                               batch, ld, st, fs);
-            hip_check(hipGetLastError(), "fft launch");
+            launch_fft<T, INV, role_direct>(is_mixed(S), G, t, batch, ld, st, fs, nblocks(batch, tpb),
+                                            (unsigned)(S.tpt * tpb), sh, s);
             return;
         }
         scratch.ensure((size_t)std::min<long long>(chunk, batch) * n * sizeof(cx<T>));
@@ -855,8 +888,7 @@ struct core_fft
         fs.n2      = (int)n2;
         fs.lbits   = lbits;
         fs.n       = (long long)n;
-        allow_lds(k_fft<T, INV, role_col>, shA);
-        allow_lds(k_fft<T, INV, role_row>, shB);
+        const bool mA = is_mixed(SA), mB = is_mixed(SB);
         for (long long b0 = 0; b0 < batch; b0 += chunk)
         {
             const long long nb = std::min(chunk, batch - b0);
@@ -866,14 +898,12 @@ struct core_fft
             sb.b0 += b0;
             const long long ga = nb * (long long)n1;
             fs.ndiv            = make_fastdiv((uint32_t)n2);
-            hipLaunchKernelGGL((k_fft<T, INV, role_col>), dim3(nblocks(ga, C)), dim3(SA.tpt * C), shA, s, GA, t + twA,
-                               ga, la, st, fs);
-            hip_check(hipGetLastError(), "four-step column pass launch");
+            launch_fft<T, INV, role_col>(mA, GA, t + twA, ga, la, st, fs, nblocks(ga, C), (unsigned)(SA.tpt * C), shA,
+                                         s);
             const long long gb = nb * (long long)n2;
             fs.ndiv            = make_fastdiv((uint32_t)n1);
-            hipLaunchKernelGGL((k_fft<T, INV, role_row>), dim3(nblocks(gb, R)), dim3(SB.tpt * R), shB, s, GB, t + twB,
-                               gb, ld, sb, fs);
-            hip_check(hipGetLastError(), "four-step row pass launch");
+            launch_fft<T, INV, role_row>(mB, GB, t + twB, gb, ld, sb, fs, nblocks(gb, R), (unsigned)(SB.tpt * R), shB,
+                                         s);
         }
     }
 

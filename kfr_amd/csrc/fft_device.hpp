@@ -310,7 +310,16 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, const fastdiv& f)
 // every HBM element is touched once.
 
 constexpr int max_passes = 20;
-constexpr int pmax       = 16; // complex values a thread holds per pass
+// Complex values a thread holds per pass and stages per load/store phase: 16 complex f32 (a radix-16
+// butterfly) or 8 complex f64 (radix 8). Kernels come in two register classes: power-of-two radices
+// only, with up to 1024 threads per block (128 VGPRs each), and mixed radices (odd butterflies and
+// direct prime passes need more temporaries), with up to 512 threads (256 VGPRs each). Both fit
+// without spilling to scratch.
+template <typename T>
+inline constexpr int pmax_of = sizeof(T) == 4 ? 16 : 8;
+constexpr int max_block_pow2  = 1024;
+constexpr int max_block_mixed = 512;
+__host__ __device__ constexpr int max_block_of(bool mixed) { return mixed ? max_block_mixed : max_block_pow2; }
 
 enum pass_kind : int
 {
@@ -336,14 +345,14 @@ struct lds_geom
     int mapping; // 0: transform = tid / tpt ; 1: transform = tid % tpb
     int ls;      // LDS element stride
     int ld;      // LDS distance between transforms
-    int pad_;
+    int padshift; // LDS padding: one slot per 2^padshift elements
     pass_desc pass[max_passes];
 };
 
 // LDS layout: logical element x lives at x + x/16, one padding slot per 16 elements, so that the
 // power-of-two strides of the Stockham passes (stride-R writes of the first pass, one transform per
 // lane for short lengths) spread over the banks instead of hitting one.
-__device__ __forceinline__ int lds_pad(int x) { return x + (x >> 4); }
+__device__ __forceinline__ int lds_pad(int x, int s) { return x + (x >> s); }
 __host__ __device__ inline size_t lds_padded_elems(size_t logical) { return logical + logical / 16 + 1; }
 
 template <typename T>
@@ -352,12 +361,13 @@ struct fft_ctx
     cx<T>* lds;  // the block's LDS
     int base;    // this transform's logical LDS offset (tb * ld)
     int ls;      // logical element stride
+    int ps;      // padding shift
     int t;       // thread index within the transform
     int tb;      // transform index within the block
     bool valid;  // transform exists (batch tail)
     long long g; // global transform index
     // element i of this transform (padded layout, see lds_pad)
-    __device__ __forceinline__ cx<T>& at(int i) const { return lds[lds_pad(base + i * ls)]; }
+    __device__ __forceinline__ cx<T>& at(int i) const { return lds[lds_pad(base + i * ls, ps)]; }
 };
 
 // one radix pass, LDS -> LDS
@@ -365,7 +375,7 @@ template <typename T, int R, bool INV>
 __device__ __forceinline__ void radix_pass(const lds_geom& G, const pass_desc& P, const cx<T>* __restrict__ tw,
                                            const fft_ctx<T>& c)
 {
-    constexpr int M = pmax / R > 0 ? pmax / R : 1;
+    constexpr int M = pmax_of<T> / R > 0 ? pmax_of<T> / R : 1;
     const int nb    = G.n / R;
     cx<T> v[M][R];
     int kk[M];
@@ -426,9 +436,9 @@ __device__ __forceinline__ void generic_pass(const lds_geom& G, const pass_desc&
     const int nb = G.n / R;
     const int M  = P.ns * R; // table length
     const cx<T>* w = tw + P.tw;
-    cx<T> out[pmax];
+    cx<T> out[pmax_of<T>];
 #pragma unroll
-    for (int m = 0; m < pmax; ++m)
+    for (int m = 0; m < pmax_of<T>; ++m)
     {
         const int o = c.t + m * G.tpt;
         out[m]      = { T(0), T(0) };
@@ -454,7 +464,7 @@ __device__ __forceinline__ void generic_pass(const lds_geom& G, const pass_desc&
     }
     __syncthreads();
 #pragma unroll
-    for (int m = 0; m < pmax; ++m)
+    for (int m = 0; m < pmax_of<T>; ++m)
     {
         const int o = c.t + m * G.tpt;
         if (o < G.n)
@@ -463,8 +473,9 @@ __device__ __forceinline__ void generic_pass(const lds_geom& G, const pass_desc&
     __syncthreads();
 }
 
-// all passes of the transform held in LDS (natural order in, natural order out)
-template <typename T, bool INV>
+// all passes of the transform held in LDS (natural order in, natural order out). MIXED = false
+// compiles only the power-of-two butterflies (radix 16 for f32, 8 for f64, and the remainders).
+template <typename T, bool INV, bool MIXED>
 __device__ __forceinline__ void fft_passes(const lds_geom& G, const cx<T>* tw, const fft_ctx<T>& c)
 {
     for (int p = 0; p < G.np; ++p)
@@ -475,26 +486,35 @@ __device__ __forceinline__ void fft_passes(const lds_geom& G, const cx<T>* tw, c
         case 2:
             radix_pass<T, 2, INV>(G, P, tw, c);
             break;
-        case 3:
-            radix_pass<T, 3, INV>(G, P, tw, c);
-            break;
         case 4:
             radix_pass<T, 4, INV>(G, P, tw, c);
-            break;
-        case 5:
-            radix_pass<T, 5, INV>(G, P, tw, c);
-            break;
-        case 7:
-            radix_pass<T, 7, INV>(G, P, tw, c);
             break;
         case 8:
             radix_pass<T, 8, INV>(G, P, tw, c);
             break;
         case 16:
-            radix_pass<T, 16, INV>(G, P, tw, c);
+            if constexpr (sizeof(T) == 4)
+                radix_pass<T, 16, INV>(G, P, tw, c);
             break;
         default:
-            generic_pass<T, INV>(G, P, tw, c);
+            if constexpr (MIXED)
+            {
+                switch (P.radix)
+                {
+                case 3:
+                    radix_pass<T, 3, INV>(G, P, tw, c);
+                    break;
+                case 5:
+                    radix_pass<T, 5, INV>(G, P, tw, c);
+                    break;
+                case 7:
+                    radix_pass<T, 7, INV>(G, P, tw, c);
+                    break;
+                default:
+                    generic_pass<T, INV>(G, P, tw, c);
+                    break;
+                }
+            }
             break;
         }
     }
@@ -520,6 +540,7 @@ __device__ __forceinline__ fft_ctx<T> make_ctx(const lds_geom& G, cx<T>* lds, lo
     c.lds   = lds;
     c.base  = c.tb * G.ld;
     c.ls    = G.ls;
+    c.ps    = G.padshift;
     return c;
 }
 

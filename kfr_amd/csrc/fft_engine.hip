@@ -84,7 +84,7 @@ std::string schedule_info::text() const
 
 // fused: both M-point transforms and the spectrum product in one workgroup, data in LDS throughout
 template <typename T, bool INV>
-__global__ void __launch_bounds__(1024) k_bluestein(lds_geom G, const cx<T>* __restrict__ tw, long long batch,
+__global__ void __launch_bounds__(max_block_pow2) k_bluestein(lds_geom G, const cx<T>* __restrict__ tw, long long batch,
                                                     const cx<T>* __restrict__ H, xload<T> ld, xstore<T> st,
                                                     fastdiv mdiv)
 {
@@ -94,11 +94,11 @@ __global__ void __launch_bounds__(1024) k_bluestein(lds_geom G, const cx<T>* __r
     const fft_ctx<T> c    = make_ctx<T>(G, lds, first, batch);
     block_load(G, lds, first, batch, ld, mdiv); // ld carries the chirp and the zero padding
     __syncthreads();
-    fft_passes<T, false>(G, tw, c);
+    fft_passes<T, false, false>(G, tw, c); // M is a power of two
     for (int i = c.t; i < G.n; i += G.tpt)
         c.at(i) = INV ? cmul(c.at(i), H[i]) : cmulc(c.at(i), H[i]);
     __syncthreads();
-    fft_passes<T, true>(G, tw, c);
+    fft_passes<T, true, false>(G, tw, c);
     block_store(G, lds, first, batch, st, mdiv); // st carries the chirp and the truncation
 }
 
@@ -197,7 +197,7 @@ struct gpu_dft<T>::impl
         if (core_fft<T>::fits_lds(M))
         {
             algo = BS_FUSED;
-            make_stockham((long long)M, bsS);
+            make_stockham((long long)M, bsS, pmax_of<T>);
             bs_tpb = core_fft<T>::pick_tpb(bsS);
             bs_tw.reset((size_t)std::max<long long>(1, bsS.tw_count) * sizeof(cx<T>));
             gen_stockham_twiddles(bsS, bs_tw.as<cx<T>>(), s);
@@ -359,10 +359,11 @@ __global__ void k_real_repack(cx<T>* out, long long odist, const cx<T>* Z, long 
 }
 
 // fused forward real transform for LDS-resident half lengths: the FFT result is repacked from LDS
-template <typename T>
-__global__ void __launch_bounds__(1024) k_real_fwd(lds_geom G, const cx<T>* __restrict__ tw, long long batch,
-                                                   xload<T> ld, cx<T>* out, long long odist,
-                                                   const cx<T>* __restrict__ rtw, int fmt, fastdiv ndiv)
+template <typename T, bool MIXED>
+__global__ void __launch_bounds__(max_block_of(MIXED)) k_real_fwd(lds_geom G, const cx<T>* __restrict__ tw,
+                                                                   long long batch, xload<T> ld, cx<T>* out,
+                                                                   long long odist, const cx<T>* __restrict__ rtw,
+                                                                   int fmt, fastdiv ndiv)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lds            = reinterpret_cast<cx<T>*>(smem_raw);
@@ -370,7 +371,7 @@ __global__ void __launch_bounds__(1024) k_real_fwd(lds_geom G, const cx<T>* __re
     const fft_ctx<T> c    = make_ctx<T>(G, lds, first, batch);
     block_load(G, lds, first, batch, ld, ndiv);
     __syncthreads();
-    fft_passes<T, false>(G, tw, c);
+    fft_passes<T, false, MIXED>(G, tw, c);
     if (!c.valid)
         return;
     const long long csize = G.n;
@@ -426,10 +427,20 @@ struct gpu_dft_real<T>::impl
             core_fft<T>& f  = *c->core;
             lds_geom G      = make_geom(f.S, f.tpb, 0, 1, (int)csize);
             const size_t sh = lds_padded_elems((size_t)f.tpb * csize) * sizeof(cx<T>);
-            allow_lds(k_real_fwd<T>, sh);
-            hipLaunchKernelGGL((k_real_fwd<T>), dim3(nblocks(batch, f.tpb)), dim3(f.S.tpt * f.tpb), sh, s, G,
-                               f.tw.template as<cx<T>>(), batch, ld, out, odist, rtw.template as<cx<T>>(), fmt,
-                               make_fastdiv((uint32_t)csize));
+            const dim3 grid(nblocks(batch, f.tpb)), block(f.S.tpt * f.tpb);
+            const fastdiv nd = make_fastdiv((uint32_t)csize);
+            if (is_mixed(f.S))
+            {
+                allow_lds(k_real_fwd<T, true>, sh);
+                hipLaunchKernelGGL((k_real_fwd<T, true>), grid, block, sh, s, G, f.tw.template as<cx<T>>(), batch, ld,
+                                   out, odist, rtw.template as<cx<T>>(), fmt, nd);
+            }
+            else
+            {
+                allow_lds(k_real_fwd<T, false>, sh);
+                hipLaunchKernelGGL((k_real_fwd<T, false>), grid, block, sh, s, G, f.tw.template as<cx<T>>(), batch, ld,
+                                   out, odist, rtw.template as<cx<T>>(), fmt, nd);
+            }
             hip_check(hipGetLastError(), "real forward launch");
             return;
         }
