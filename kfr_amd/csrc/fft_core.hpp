@@ -11,6 +11,7 @@
 
 #include "fft_device.hpp"
 #include "fft_engine.hpp"
+#include "fft_pow2.hpp"
 
 #include <algorithm>
 #include <climits>
@@ -252,7 +253,8 @@ enum twiddle_mode : int
 {
     tw_linear = 0, // a = (i*mult + add) mod M
     tw_radix  = 1, // radix-pass table: i = k*(R-1) + (r-1), a = r*k; M = Ns*R
-    tw_square = 2  // a = i^2 mod M (Bluestein chirp, M = 2N)
+    tw_square = 2, // a = i^2 mod M (Bluestein chirp, M = 2N)
+    tw_radix_full = 3 // radix-pass table with r = 0 included: i = k*R + r, a = r*k; M = Ns*R
 };
 
 template <typename T>
@@ -263,7 +265,13 @@ __global__ void k_twiddles(cx<T>* out, long long count, long long M, long long m
     if (i >= count)
         return;
     unsigned long long a;
-    if (mode == tw_radix)
+    if (mode == tw_radix_full)
+    {
+        const long long k = i / R;
+        const long long r = i - k * R;
+        a                 = (unsigned long long)((r * k) % M);
+    }
+    else if (mode == tw_radix)
     {
         const long long k = i / (R - 1);
         const long long r = i - k * (R - 1) + 1;
@@ -744,6 +752,8 @@ struct core_fft
     dev_mem tw;
     long long twA = 0, twB = 0, twTA = 0, twTB = 0;
     dev_mem scratch;
+    // power-of-two lengths: specialised kernels for contiguous batches (plain loads and stores)
+    std::unique_ptr<pow2_fft<T>> pow2;
 
     static bool fits_lds(size_t len)
     {
@@ -767,6 +777,8 @@ struct core_fft
     core_fft(size_t len, hipStream_t s)
     {
         n = len;
+        if (pow2_fft<T>::supports(n))
+            pow2.reset(new pow2_fft<T>(n, s));
         if (fits_lds(n))
         {
             make_stockham((long long)n, S, pmax_of<T>);
@@ -862,6 +874,12 @@ struct core_fft
     {
         if (batch <= 0)
             return;
+        if (pow2 && plain_batch(ld) && plain_batch(st))
+        {
+            pow2->run(INV, static_cast<const cx<T>*>(ld.p) + ld.b0 * (long long)n,
+                      static_cast<cx<T>*>(st.p) + st.b0 * (long long)n, batch, s);
+            return;
+        }
         const cx<T>* t = tw.as<cx<T>>();
         if (!fourstep)
         {
@@ -905,6 +923,16 @@ struct core_fft
             launch_fft<T, INV, role_row>(mB, GB, t + twB, gb, ld, sb, fs, nblocks(gb, R), (unsigned)(SB.tpt * R), shB,
                                          s);
         }
+    }
+
+    // contiguous batch: item b at b*n, no element operation
+    bool plain_batch(const xload<T>& l) const
+    {
+        return l.mode == ld_plain && l.pdist == (long long)n && l.mtab == nullptr && l.lim == LLONG_MAX;
+    }
+    bool plain_batch(const xstore<T>& st) const
+    {
+        return st.mode == st_plain && st.pdist == (long long)n && st.mtab == nullptr && st.lim == LLONG_MAX;
     }
 
     std::vector<int> radices() const

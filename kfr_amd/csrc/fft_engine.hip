@@ -73,6 +73,8 @@ std::string schedule_info::text() const
     if (threads_per_transform)
         s += " threads/transform=" + std::to_string(threads_per_transform) +
              " transforms/block=" + std::to_string(transforms_per_block);
+    if (!contiguous.empty())
+        s += "; contiguous batches: " + contiguous;
     return s;
 }
 
@@ -164,6 +166,8 @@ struct gpu_dft<T>::impl
                 info.threads_per_transform = core->S.tpt;
                 info.transforms_per_block  = core->tpb;
             }
+            if (core->pow2)
+                info.contiguous = core->pow2->describe();
         }
         else
             init_bluestein(s);

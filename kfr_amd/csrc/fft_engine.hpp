@@ -58,6 +58,7 @@ struct schedule_info
     size_t bluestein_m = 0;        // Bluestein FFT length
     int threads_per_transform = 0; // lds
     int transforms_per_block  = 0; // lds
+    std::string contiguous;        // kernels that contiguous batches use instead, when they differ
     std::string text() const;
 };
 

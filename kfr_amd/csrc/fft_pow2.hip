@@ -1,0 +1,542 @@
+// Power-of-two C2C kernels specialised per length.
+//
+// A transform of N = 2^L points is split over TPT = N/E threads (E = 16 points per thread for f32,
+// 8 for f64). Every pass of the Stockham autosort schedule reads, in thread t, the strided set
+// x[t + k*TPT] (k < E), so the first pass takes its operands straight from HBM (coalesced across
+// threads), the passes exchange data through LDS, and the last pass, whose outputs are the same
+// strided set in natural order, writes straight to HBM. Radix-E passes (16 or 8) follow one
+// smaller power-of-two pass that absorbs L mod log2(E); with the radices, pass strides, table
+// offsets and loop bounds all compile-time constants, the butterflies run from registers.
+//
+// Lengths beyond one workgroup's LDS use the four-step split n = n1*n2 with x[c + n1*i]:
+// kernel "col" transforms the n2-point columns (W columns per workgroup, consecutive lanes on
+// consecutive columns), multiplies by W_n^(c*k2) and writes the intermediate; kernel "row"
+// transforms the n1-point rows and stores them transposed through LDS, so both kernels move
+// W-element runs. Workgroups of both kernels are ordered so that neighbouring column (or row)
+// groups land on the same XCD and share its L2.
+#include "fft_pow2.hpp"
+
+#include "fft_core.hpp"
+
+#include <stdexcept>
+
+namespace kfr_amd
+{
+namespace p2
+{
+
+template <typename T>
+inline constexpr int log_e = sizeof(T) == 4 ? 4 : 3; // log2 points per thread
+template <typename T>
+inline constexpr int max_log = sizeof(T) == 4 ? 14 : 13; // longest single-kernel transform
+template <typename T>
+inline constexpr int lds_budget = sizeof(T) == 4 ? 16384 : 8192; // complex values per workgroup
+constexpr int min_split_log = 7;                                 // shortest four-step factor
+
+// pass schedule of 2^L points (also used on the host to lay out the twiddle tables)
+__host__ __device__ constexpr int pass_le(int L, int le) { return L < le ? L : le; }
+__host__ __device__ constexpr int pass_rem(int L, int le) { return L % pass_le(L, le); }
+__host__ __device__ constexpr int pass_count(int L, int le)
+{
+    return L / pass_le(L, le) + (pass_rem(L, le) ? 1 : 0);
+}
+__host__ __device__ constexpr int pass_lr(int L, int le, int p) // log2 radix of pass p
+{
+    return (pass_rem(L, le) && p == 0) ? pass_rem(L, le) : pass_le(L, le);
+}
+__host__ __device__ constexpr int pass_lns(int L, int le, int p) // log2 of the stride Ns of pass p
+{
+    int s = 0;
+    for (int q = 0; q < p; ++q)
+        s += pass_lr(L, le, q);
+    return s;
+}
+// pass p's table holds W_{Ns*R}^(r*k) at k*R + r (k < Ns, r < R); passes with Ns = 1 have none
+__host__ __device__ constexpr int pass_twoff(int L, int le, int p)
+{
+    int o = 0;
+    for (int q = 0; q < p; ++q)
+        if (pass_lns(L, le, q) > 0)
+            o += 1 << (pass_lns(L, le, q) + pass_lr(L, le, q));
+    return o;
+}
+
+template <typename T, int L>
+struct shape
+{
+    static constexpr int N   = 1 << L;
+    static constexpr int LE  = pass_le(L, log_e<T>);
+    static constexpr int E   = 1 << LE;
+    static constexpr int TPT = N / E;
+    static constexpr int NP  = pass_count(L, log_e<T>);
+    static constexpr int lr(int p) { return pass_lr(L, log_e<T>, p); }
+    static constexpr int lns(int p) { return pass_lns(L, log_e<T>, p); }
+    static constexpr int twoff(int p) { return pass_twoff(L, log_e<T>, p); }
+    static constexpr int twcount = pass_twoff(L, log_e<T>, pass_count(L, log_e<T>));
+};
+
+// the values one thread holds
+template <typename T, int L>
+using regs = cx<T>[shape<T, L>::E];
+
+template <typename T, int L, int ROLE>
+struct geom
+{
+    using S = shape<T, L>;
+    static constexpr int width()
+    {
+        if (ROLE == role_direct)
+            return S::TPT >= 256 ? 1 : 256 / S::TPT;
+        int w = 16;
+        while (w > 1 && (w * S::N > lds_budget<T> || w * S::TPT > 1024))
+            w /= 2;
+        return w;
+    }
+    static constexpr int W       = width(); // transforms per workgroup
+    static constexpr int BS      = S::TPT * W;
+    static constexpr int LD      = ROLE == role_row ? S::N + 1 : S::N; // LDS distance between transforms
+    static constexpr int logical = ROLE == role_row ? W * LD : W * S::N;
+    static constexpr size_t lds_bytes = (size_t)(logical + logical / 16 + 1) * sizeof(cx<T>);
+    // short transforms (under 16 threads each) load and store through LDS in whole-workgroup runs
+    static constexpr bool staged = ROLE == role_direct && S::TPT < 16;
+    static_assert(BS <= 1024, "workgroup too large");
+};
+
+// LDS slot of element i of the workgroup's transform tb: columns are interleaved (consecutive lanes
+// on consecutive columns), other roles keep each transform contiguous; one padding slot per 16
+// values spreads the power-of-two strides of the passes over the banks.
+template <int ROLE, int W, int LD>
+__device__ __forceinline__ int laddr(int tb, int i)
+{
+    const int x = ROLE == role_col ? i * W + tb : tb * LD + i;
+    return x + (x >> 4);
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void load_table(const cx<T>* p, cx<T> (&w)[R])
+{
+    if constexpr (sizeof(T) == 4 && R >= 2)
+    {
+        const float4* q = reinterpret_cast<const float4*>(p);
+#pragma unroll
+        for (int i = 0; i < R / 2; ++i)
+        {
+            const float4 f = q[i];
+            w[2 * i]       = { f.x, f.y };
+            w[2 * i + 1]   = { f.z, f.w };
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+            w[i] = p[i];
+    }
+}
+
+template <typename T>
+struct args
+{
+    const cx<T>* in  = nullptr;
+    cx<T>* out       = nullptr;
+    const cx<T>* tw  = nullptr; // pass tables of this kernel's length
+    const cx<T>* ta  = nullptr; // four-step: W_n^(h*2^lbits)
+    const cx<T>* tb  = nullptr; // four-step: W_n^l, l < 2^lbits
+    long long count  = 0;       // transforms
+    int lo           = 0;       // four-step: log2 of the other dimension (col: n1, row: n2)
+    int lbits        = 0;
+    unsigned nblk    = 0; // workgroups with work (the grid of split kernels is a multiple of 8)
+};
+
+// pass P of the schedule on the registers v (strided set in, strided set out after the exchange)
+template <typename T, int L, bool INV, int ROLE, int P>
+__device__ __forceinline__ void run_pass(regs<T, L>& v, cx<T>* lds, int tb, int t, const cx<T>* tw)
+{
+    using S          = shape<T, L>;
+    using G          = geom<T, L, ROLE>;
+    constexpr int E  = S::E;
+    constexpr int TP = S::TPT;
+    constexpr int R  = 1 << S::lr(P);
+    constexpr int NS = 1 << S::lns(P);
+    constexpr int M  = E / R;
+    // v[m + r*M] holds x[j + r*N/R] for butterfly j = t + m*TPT
+    if constexpr (NS > 1)
+    {
+#pragma unroll
+        for (int m = 0; m < M; ++m)
+        {
+            const int j  = t + m * TP;
+            const int kt = j & (NS - 1);
+            cx<T> w[R];
+            load_table<T, R>(tw + S::twoff(P) + kt * R, w);
+#pragma unroll
+            for (int r = 1; r < R; ++r)
+                v[m + r * M] = ctw<INV>(v[m + r * M], w[r]);
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < M; ++m)
+    {
+        cx<T> u[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            u[r] = v[m + r * M];
+        butterfly<R, INV>(u);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            v[m + r * M] = u[r];
+    }
+    if constexpr (P + 1 < S::NP)
+    {
+        // output r of butterfly j goes to (j - j mod Ns)*R + j mod Ns + r*Ns
+#pragma unroll
+        for (int m = 0; m < M; ++m)
+        {
+            const int j    = t + m * TP;
+            const int kt   = j & (NS - 1);
+            const int base = (j - kt) * R + kt;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                lds[laddr<ROLE, G::W, G::LD>(tb, base + r * NS)] = v[m + r * M];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < E; ++k)
+            v[k] = lds[laddr<ROLE, G::W, G::LD>(tb, t + k * TP)];
+        __syncthreads();
+        run_pass<T, L, INV, ROLE, P + 1>(v, lds, tb, t, tw);
+    }
+}
+
+template <typename T, int L, bool INV, int ROLE>
+__global__ void __launch_bounds__((geom<T, L, ROLE>::BS)) k_pow2(args<T> a)
+{
+    using S          = shape<T, L>;
+    using G          = geom<T, L, ROLE>;
+    constexpr int N  = S::N;
+    constexpr int E  = S::E;
+    constexpr int TP = S::TPT;
+    constexpr int W  = G::W;
+    constexpr int BS = G::BS;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+
+    unsigned blk = blockIdx.x;
+    if constexpr (ROLE != role_direct)
+    {
+        // workgroup b runs on XCD b mod 8: give each XCD a contiguous range of column/row groups
+        const unsigned per = gridDim.x / 8;
+        blk                = (blockIdx.x % 8) * per + blockIdx.x / 8;
+        if (blk >= a.nblk)
+            return; // whole workgroup, before any barrier
+    }
+    const int tid     = threadIdx.x;
+    const int tb      = ROLE == role_col ? tid % W : tid / TP;
+    const int t       = ROLE == role_col ? tid / W : tid % TP;
+    const long long g = (long long)blk * W + tb;
+    const bool valid  = g < a.count;
+    cx<T> v[E];
+
+    if constexpr (ROLE == role_direct && G::staged)
+    {
+        const long long base = (long long)blk * W * N;
+        const long long lim  = a.count * N - base;
+#pragma unroll
+        for (int k = 0; k < E; ++k)
+        {
+            const int e = tid + k * BS;
+            v[k]        = e < lim ? a.in[base + e] : cx<T>{ T(0), T(0) };
+        }
+#pragma unroll
+        for (int k = 0; k < E; ++k)
+        {
+            const int e    = tid + k * BS;
+            lds[e + (e >> 4)] = v[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < E; ++k)
+            v[k] = lds[laddr<ROLE, W, G::LD>(tb, t + k * TP)];
+        __syncthreads();
+    }
+    else if constexpr (ROLE == role_col)
+    {
+        const long long bl  = g >> a.lo;
+        const long long col = g & ((1ll << a.lo) - 1);
+        const cx<T>* src    = a.in + (bl << (a.lo + L)) + col;
+#pragma unroll
+        for (int k = 0; k < E; ++k)
+            v[k] = valid ? src[(long long)(t + k * TP) << a.lo] : cx<T>{ T(0), T(0) };
+    }
+    else
+    {
+        const cx<T>* src = a.in + g * N + t;
+#pragma unroll
+        for (int k = 0; k < E; ++k)
+            v[k] = valid ? src[k * TP] : cx<T>{ T(0), T(0) };
+    }
+
+    run_pass<T, L, INV, ROLE, 0>(v, lds, tb, t, a.tw);
+    // v[k] = X[t + k*TPT]
+
+    if constexpr (ROLE == role_direct && G::staged)
+    {
+#pragma unroll
+        for (int k = 0; k < E; ++k)
+            lds[laddr<ROLE, W, G::LD>(tb, t + k * TP)] = v[k];
+        __syncthreads();
+        const long long base = (long long)blk * W * N;
+        const long long lim  = a.count * N - base;
+#pragma unroll
+        for (int k = 0; k < E; ++k)
+        {
+            const int e = tid + k * BS;
+            if (e < lim)
+                a.out[base + e] = lds[e + (e >> 4)];
+        }
+    }
+    else if constexpr (ROLE == role_direct)
+    {
+        if (valid)
+        {
+            cx<T>* dst = a.out + g * N + t;
+#pragma unroll
+            for (int k = 0; k < E; ++k)
+                dst[k * TP] = v[k];
+        }
+    }
+    else if constexpr (ROLE == role_col)
+    {
+        if (valid)
+        {
+            const long long bl            = g >> a.lo;
+            const long long col           = g & ((1ll << a.lo) - 1);
+            cx<T>* dst                    = a.out + (bl << (a.lo + L)) + col;
+            const unsigned long long mask = (1ull << a.lbits) - 1;
+#pragma unroll
+            for (int k = 0; k < E; ++k)
+            {
+                const int k2               = t + k * TP;
+                const unsigned long long m = (unsigned long long)col * (unsigned long long)k2; // < n
+                const cx<T> w              = cmul(a.ta[m >> a.lbits], a.tb[m & mask]);
+                dst[(long long)k2 << a.lo] = ctw<INV>(v[k], w);
+            }
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int k = 0; k < E; ++k)
+            lds[laddr<ROLE, W, G::LD>(tb, t + k * TP)] = v[k];
+        __syncthreads();
+        // transposed store: row index fastest, so lanes write W consecutive outputs k2 + n2*k1
+        const int tb2      = tid % W;
+        const int i0       = tid / W;
+        const long long g2 = (long long)blk * W + tb2;
+        if (g2 < a.count)
+        {
+            const long long bl = g2 >> a.lo;
+            const long long k2 = g2 & ((1ll << a.lo) - 1);
+            cx<T>* dst         = a.out + (bl << (a.lo + L)) + k2;
+#pragma unroll
+            for (int k = 0; k < E; ++k)
+            {
+                const int i                = i0 + k * TP;
+                dst[(long long)i << a.lo] = lds[laddr<ROLE, W, G::LD>(tb2, i)];
+            }
+        }
+    }
+}
+
+template <typename T, int L, bool INV, int ROLE>
+void launch(args<T> a, long long count, hipStream_t s)
+{
+    using G                  = geom<T, L, ROLE>;
+    static const bool lds_ok = (allow_lds(k_pow2<T, L, INV, ROLE>, G::lds_bytes), true);
+    (void)lds_ok;
+    if (count <= 0)
+        return;
+    a.count             = count;
+    const long long nb  = (count + G::W - 1) / G::W;
+    const long long grd = ROLE == role_direct ? nb : (nb + 7) / 8 * 8;
+    if (grd > 0x7fffffffll)
+        throw std::runtime_error("batch too large for one launch");
+    a.nblk = (unsigned)nb;
+    hipLaunchKernelGGL((k_pow2<T, L, INV, ROLE>), dim3((unsigned)grd), dim3(G::BS), G::lds_bytes, s, a);
+    hip_check(hipGetLastError(), "power-of-two FFT launch");
+}
+
+template <typename T, bool INV, int ROLE, int LO, int HI>
+void launch_len(int L, const args<T>& a, long long count, hipStream_t s)
+{
+    if constexpr (LO <= HI)
+    {
+        if (L == LO)
+            return launch<T, LO, INV, ROLE>(a, count, s);
+        return launch_len<T, INV, ROLE, LO + 1, HI>(L, a, count, s);
+    }
+    else
+        throw std::logic_error("no power-of-two kernel for this length");
+}
+
+template <typename T, int ROLE, int LO, int HI>
+void launch_any(bool inv, int L, const args<T>& a, long long count, hipStream_t s)
+{
+    if (inv)
+        launch_len<T, true, ROLE, LO, HI>(L, a, count, s);
+    else
+        launch_len<T, false, ROLE, LO, HI>(L, a, count, s);
+}
+
+// pass tables of 2^L points (layout of pass_twoff), FP64-generated
+template <typename T>
+void gen_pass_tables(int L, cx<T>* tw, hipStream_t s)
+{
+    const int le = log_e<T>;
+    for (int p = 0; p < pass_count(L, le); ++p)
+    {
+        const long long ns = 1ll << pass_lns(L, le, p);
+        const int r        = 1 << pass_lr(L, le, p);
+        if (ns > 1)
+            gen_twiddles(tw + pass_twoff(L, le, p), ns * r, ns * r, 0, 0, tw_radix_full, r, s);
+    }
+}
+
+inline int log2_exact(size_t n)
+{
+    if (n == 0 || (n & (n - 1)))
+        return -1;
+    int l = 0;
+    while ((size_t(1) << l) < n)
+        ++l;
+    return l;
+}
+
+} // namespace p2
+
+template <typename T>
+struct pow2_fft<T>::impl
+{
+    int L     = 0;
+    bool four = false;
+    int L1 = 0, L2 = 0; // four-step: rows of n1 = 2^L1 points, columns of n2 = 2^L2 points
+    int lbits = 0;
+    dev_mem tw;
+    long long o_row = 0, o_ta = 0, o_tb = 0;
+    dev_mem scratch;
+    long long chunk = 1;
+};
+
+template <typename T>
+bool pow2_fft<T>::supports(size_t n)
+{
+    const int L = p2::log2_exact(n);
+    return L >= 1 && L <= 2 * p2::max_log<T>;
+}
+
+template <typename T>
+pow2_fft<T>::pow2_fft(size_t n, hipStream_t s) : p(new impl)
+{
+    const int L = p2::log2_exact(n);
+    if (!supports(n))
+        throw std::invalid_argument("pow2_fft: unsupported length");
+    const int le = p2::log_e<T>;
+    p->L         = L;
+    if (L <= p2::max_log<T>)
+    {
+        const int cnt = p2::pass_twoff(L, le, p2::pass_count(L, le));
+        p->tw.reset((size_t)std::max(1, cnt) * sizeof(cx<T>));
+        p2::gen_pass_tables<T>(L, p->tw.template as<cx<T>>(), s);
+        return;
+    }
+    p->four              = true;
+    p->L2                = std::max(p2::min_split_log, L / 2);
+    p->L1                = L - p->L2;
+    const int c2         = p2::pass_twoff(p->L2, le, p2::pass_count(p->L2, le));
+    const int c1         = p2::pass_twoff(p->L1, le, p2::pass_count(p->L1, le));
+    p->lbits             = (L + 1) / 2;
+    const long long nTB  = 1ll << p->lbits;
+    const long long nTA  = (long long)(n >> p->lbits) + 1;
+    p->o_row             = (c2 + 1) / 2 * 2; // keep every table 16-byte aligned
+    p->o_ta              = p->o_row + (c1 + 1) / 2 * 2;
+    p->o_tb              = p->o_ta + (nTA + 1) / 2 * 2;
+    p->tw.reset((size_t)(p->o_tb + nTB) * sizeof(cx<T>));
+    cx<T>* t = p->tw.template as<cx<T>>();
+    p2::gen_pass_tables<T>(p->L2, t, s);
+    p2::gen_pass_tables<T>(p->L1, t + p->o_row, s);
+    gen_twiddles(t + p->o_ta, nTA, (long long)n, nTB, 0, tw_linear, 0, s);
+    gen_twiddles(t + p->o_tb, nTB, (long long)n, 1, 0, tw_linear, 0, s);
+    // the intermediate of one round stays within the 256 MB last-level (Infinity) cache
+    size_t cap = (size_t)128 << 20;
+    if (const char* e = std::getenv("KFR_AMD_FOURSTEP_CHUNK_MB"))
+        cap = (size_t)std::max(1, atoi(e)) << 20;
+    p->chunk = (long long)std::max<size_t>(1, cap / (n * sizeof(cx<T>)));
+}
+
+template <typename T>
+pow2_fft<T>::~pow2_fft() = default;
+
+template <typename T>
+void pow2_fft<T>::run(bool inverse, const void* in, void* out, long long batch, hipStream_t s)
+{
+    if (batch <= 0)
+        return;
+    using p2::args;
+    constexpr int ML = p2::max_log<T>;
+    const cx<T>* x   = static_cast<const cx<T>*>(in);
+    cx<T>* y         = static_cast<cx<T>*>(out);
+    const cx<T>* t   = p->tw.template as<cx<T>>();
+    if (!p->four)
+    {
+        args<T> a;
+        a.in  = x;
+        a.out = y;
+        a.tw  = t;
+        p2::launch_any<T, role_direct, 1, ML>(inverse, p->L, a, batch, s);
+        return;
+    }
+    const long long n = 1ll << p->L;
+    p->scratch.ensure((size_t)std::min(p->chunk, batch) * n * sizeof(cx<T>));
+    cx<T>* z = p->scratch.template as<cx<T>>();
+    for (long long b0 = 0; b0 < batch; b0 += p->chunk)
+    {
+        const long long nb = std::min(p->chunk, batch - b0);
+        args<T> a;
+        a.ta    = t + p->o_ta;
+        a.tb    = t + p->o_tb;
+        a.lbits = p->lbits;
+        a.in    = x + b0 * n;
+        a.out   = z;
+        a.tw    = t;
+        a.lo    = p->L1;
+        p2::launch_any<T, role_col, p2::min_split_log, ML>(inverse, p->L2, a, nb << p->L1, s);
+        a.in  = z;
+        a.out = y + b0 * n;
+        a.tw  = t + p->o_row;
+        a.lo  = p->L2;
+        p2::launch_any<T, role_row, p2::min_split_log, ML>(inverse, p->L1, a, nb << p->L2, s);
+    }
+}
+
+template <typename T>
+std::string pow2_fft<T>::describe() const
+{
+    const int le = p2::log_e<T>;
+    auto radices = [&](int L)
+    {
+        std::string r;
+        for (int q = 0; q < p2::pass_count(L, le); ++q)
+            r += (q ? "x" : "") + std::to_string(1 << p2::pass_lr(L, le, q));
+        return r;
+    };
+    if (!p->four)
+        return "pow2 radices=" + radices(p->L) + " threads/transform=" +
+               std::to_string((1 << p->L) >> p2::pass_le(p->L, le));
+    return "pow2 four-step n1=" + std::to_string(1ll << p->L1) + " n2=" + std::to_string(1ll << p->L2) +
+           " radices=" + radices(p->L2) + "|" + radices(p->L1) + " chunk=" + std::to_string(p->chunk);
+}
+
+template class pow2_fft<float>;
+template class pow2_fft<double>;
+
+} // namespace kfr_amd

@@ -1,0 +1,77 @@
+"""Power-of-two C2C kernels (one workgroup per transform group, and the four-step split) over
+contiguous device batches: every length 2^1..2^22, batch tails, in-place, both directions (gpu)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import kfr_amd
+import oracle
+from tests.common import fp64_dft, tol_kfr
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch_for(L):
+    # odd batch sizes leave a partial last workgroup; about 2^20 values per case
+    return max(1, (1 << 20) >> L) + 1 if L <= 18 else 2
+
+
+@pytest.mark.parametrize("L", list(range(1, 23)))
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_pow2_batch(L, dtype):
+    torch = pytest.importorskip("torch")
+    n = 1 << L
+    batch = _batch_for(L)
+    x = oracle.complex_uniform(900 + L, (batch, n), dtype)
+    rdt = np.float32 if dtype == np.complex64 else np.float64
+    xd = torch.from_numpy(x.view(rdt).copy()).cuda()
+    yd = torch.empty_like(xd)
+    plan = kfr_amd.dft_plan(n, dtype)
+    sched = kfr_amd.schedule("f32" if dtype == np.complex64 else "f64", n)
+    bound = max(1e-6 if dtype == np.complex64 else 1e-14, tol_kfr(n, dtype))
+    ref = {inv: fp64_dft(x.astype(np.complex128), inv) for inv in (False, True)}
+    for inverse in (False, True):
+        plan.execute_batch(yd, xd, batch, inverse)
+        torch.cuda.synchronize()
+        y = yd.cpu().numpy().view(dtype).reshape(batch, n)
+        errs = [oracle.rel_rms(y[b], ref[inverse][b]) for b in range(batch)]
+        print(f"L={L} {np.dtype(dtype).name} batch={batch} inv={inverse} worst={max(errs):.3e} {sched}", flush=True)
+        assert max(errs) <= bound, (L, inverse, max(errs), int(np.argmax(errs)))
+    # in place, forward
+    plan.execute_batch(xd, xd, batch, False)
+    torch.cuda.synchronize()
+    y = xd.cpu().numpy().view(dtype).reshape(batch, n)
+    assert max(oracle.rel_rms(y[b], ref[False][b]) for b in range(batch)) <= bound
+
+
+def test_pow2_untouched_tail():
+    """a batch that ends inside a workgroup writes nothing past its last transform"""
+    torch = pytest.importorskip("torch")
+    n, batch = 64, 5                       # 64-point f32 transforms run 64 to a workgroup
+    x = oracle.complex_uniform(17, (batch, n), np.complex64)
+    xd = torch.from_numpy(x.view(np.float32).copy()).cuda()
+    guard = torch.full((2 * n * (batch + 3),), 7.0, device="cuda")
+    plan = kfr_amd.dft_plan(n, np.complex64)
+    plan.execute_batch(guard, xd, batch, False)
+    torch.cuda.synchronize()
+    g = guard.cpu().numpy()
+    assert np.all(g[2 * n * batch:] == 7.0)
+    y = g[:2 * n * batch].view(np.complex64).reshape(batch, n)
+    assert oracle.rel_rms(y, fp64_dft(x.astype(np.complex128))) <= 1e-6
+
+
+def test_pow2_f64_four_step_round_trip():
+    torch = pytest.importorskip("torch")
+    n = 1 << 24
+    x = oracle.complex_uniform(23, n, np.complex128)
+    plan = kfr_amd.dft_plan(n, np.complex128)
+    xd = torch.from_numpy(x.view(np.float64).copy()).cuda()
+    y = torch.empty_like(xd)
+    plan.execute_batch(y, xd, 1, False)
+    plan.execute_batch(y, y, 1, True)
+    torch.cuda.synchronize()
+    z = y.cpu().numpy().view(np.complex128) / n
+    err = oracle.rel_rms(z, x)
+    print(f"f64 2^24 round trip rel_rms={err:.3e}", flush=True)
+    assert err < 1e-14
