@@ -79,6 +79,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true", help="sizes up to 2^20 only, C2C only")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first run one plain 256 MiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding is a design note only",
@@ -117,6 +119,13 @@ def main() -> int:
     for c in cfgs:
         assert in_bytes_of(c) <= in_bytes, c
     dst = torch.empty((out_bytes + 3) // 4, device=dev, dtype=torch.float32)
+
+    if args.copy_probe:
+        probe_src = src32[: (256 << 20) // 4]
+        probe_dst = torch.empty_like(probe_src)
+        probe_dst.copy_(probe_src)
+        torch.cuda.synchronize()
+        del probe_dst
 
     def run_cfg(c):
         kind, dt, n, batch = c
