@@ -79,6 +79,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true", help="sizes up to 2^20 only, C2C only")
+    ap.add_argument("--only", default=None,
+                    help="comma-separated subset of configurations kind:dtype:n (e.g. c2c:f32:4096), for profiling")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one plain 256 MiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -95,6 +97,12 @@ def main() -> int:
     torch.cuda.set_device(dev)
     stream = torch.cuda.current_stream()
     cfgs = sweep_configs(args.quick)
+    if args.only:
+        keep = {tuple(x.split(":")) for x in args.only.split(",")}
+        cfgs = [c for c in cfgs if (c[0], c[1], str(c[2])) in keep]
+        if not cfgs:
+            print(json.dumps({"error": f"no configuration matches --only {args.only}"}))
+            return 2
 
     # plans (excluded from timing)
     plans = {}

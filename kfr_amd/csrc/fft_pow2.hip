@@ -148,6 +148,15 @@ struct args
     unsigned nblk    = 0; // workgroups with work (the grid of split kernels is a multiple of 8)
 };
 
+// Workgroup barrier for the LDS exchanges: the fences cover the local address space only, so it
+// does not wait for outstanding global memory operations.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // pass P of the schedule on the registers v (strided set in, strided set out after the exchange)
 template <typename T, int L, bool INV, int ROLE, int P>
 __device__ __forceinline__ void run_pass(regs<T, L>& v, cx<T>* lds, int tb, int t, const cx<T>* tw)
@@ -199,11 +208,11 @@ __device__ __forceinline__ void run_pass(regs<T, L>& v, cx<T>* lds, int tb, int 
             for (int r = 0; r < R; ++r)
                 lds[laddr<ROLE, G::W, G::LD>(tb, base + r * NS)] = v[m + r * M];
         }
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int k = 0; k < E; ++k)
             v[k] = lds[laddr<ROLE, G::W, G::LD>(tb, t + k * TP)];
-        __syncthreads();
+        lds_barrier();
         run_pass<T, L, INV, ROLE, P + 1>(v, lds, tb, t, tw);
     }
 }
@@ -253,11 +262,11 @@ __global__ void __launch_bounds__((geom<T, L, ROLE>::BS)) k_pow2(args<T> a)
             const int e    = tid + k * BS;
             lds[e + (e >> 4)] = v[k];
         }
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int k = 0; k < E; ++k)
             v[k] = lds[laddr<ROLE, W, G::LD>(tb, t + k * TP)];
-        __syncthreads();
+        lds_barrier();
     }
     else if constexpr (ROLE == role_col)
     {
@@ -284,7 +293,7 @@ __global__ void __launch_bounds__((geom<T, L, ROLE>::BS)) k_pow2(args<T> a)
 #pragma unroll
         for (int k = 0; k < E; ++k)
             lds[laddr<ROLE, W, G::LD>(tb, t + k * TP)] = v[k];
-        __syncthreads();
+        lds_barrier();
         const long long base = (long long)blk * W * N;
         const long long lim  = a.count * N - base;
 #pragma unroll
@@ -328,7 +337,7 @@ __global__ void __launch_bounds__((geom<T, L, ROLE>::BS)) k_pow2(args<T> a)
 #pragma unroll
         for (int k = 0; k < E; ++k)
             lds[laddr<ROLE, W, G::LD>(tb, t + k * TP)] = v[k];
-        __syncthreads();
+        lds_barrier();
         // transposed store: row index fastest, so lanes write W consecutive outputs k2 + n2*k1
         const int tb2      = tid % W;
         const int i0       = tid / W;
@@ -466,8 +475,9 @@ pow2_fft<T>::pow2_fft(size_t n, hipStream_t s) : p(new impl)
     p2::gen_pass_tables<T>(p->L1, t + p->o_row, s);
     gen_twiddles(t + p->o_ta, nTA, (long long)n, nTB, 0, tw_linear, 0, s);
     gen_twiddles(t + p->o_tb, nTB, (long long)n, 1, 0, tw_linear, 0, s);
-    // the intermediate of one round stays within the 256 MB last-level (Infinity) cache
-    size_t cap = (size_t)128 << 20;
+    // rounds of at most 512 MiB of intermediate: few launches (measured faster than rounds small
+    // enough to stay in the last-level cache), bounded scratch
+    size_t cap = (size_t)512 << 20;
     if (const char* e = std::getenv("KFR_AMD_FOURSTEP_CHUNK_MB"))
         cap = (size_t)std::max(1, atoi(e)) << 20;
     p->chunk = (long long)std::max<size_t>(1, cap / (n * sizeof(cx<T>)));
