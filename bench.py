@@ -72,6 +72,86 @@ def alg_bytes(kind, dt, n, batch):
     return 2 * es * n * batch
 
 
+def uneven_lengths() -> np.ndarray:
+    """Every 7-smooth N in [18, 8192] and every prime in [17, 127] (328 lengths)."""
+    smooth = {2 ** a * 3 ** b * 5 ** c * 7 ** d for a in range(14) for b in range(9) for c in range(6)
+              for d in range(5)}
+    smooth = {v for v in smooth if 18 <= v <= 8192}
+    primes = [p for p in range(17, 128) if all(p % q for q in range(2, int(p ** 0.5) + 1))]
+    return np.array(sorted(smooth | set(primes)), dtype=np.int64)
+
+
+def uneven_batch(torch, kfr_amd, stream, reps: int, check: bool):
+    """20,000 complex-f32 transforms with lengths drawn uniformly from uneven_lengths(), each with its
+    own direction (Bernoulli 1/2), data U[-1,1) complex from a seed: one autotuned persistent graph
+    launch against one plain-path launch per distinct (length, direction) on size-grouped data."""
+    rng = np.random.default_rng(3)
+    L = uneven_lengths()
+    sizes = rng.choice(L, 20000)
+    inv = rng.integers(0, 2, sizes.size).astype(np.int32)
+    total = int(sizes.sum())
+    x = (rng.uniform(-1, 1, 2 * total).astype(np.float32)).view(np.complex64)
+    xd = torch.from_numpy(x.view(np.float32)).cuda()
+    yd = torch.empty_like(xd)
+    t0 = time.perf_counter()
+    b = kfr_amd.dft_batch(sizes, inv, np.complex64, autotune=True)
+    plan_s = time.perf_counter() - t0
+    # per-size path: transforms grouped by (length, direction), contiguous per group
+    groups = {}
+    for i, (n, d) in enumerate(zip(sizes, inv)):
+        groups.setdefault((int(n), int(d)), []).append(i)
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    order = [i for k in sorted(groups) for i in groups[k]]
+    xg = np.concatenate([x[offs[i]:offs[i] + sizes[i]] for i in order])
+    xgd = torch.from_numpy(xg.view(np.float32)).cuda()
+    ygd = torch.empty_like(xgd)
+    plans = {n: kfr_amd.dft_plan(n, np.complex64) for (n, _) in groups}
+    runs = []
+    pos = 0
+    for (n, d) in sorted(groups):
+        cnt = len(groups[(n, d)])
+        runs.append((plans[n], pos, n, cnt, bool(d)))
+        pos += n * cnt
+
+    def per_size():
+        for p, o, n, cnt, d in runs:
+            p.execute_batch(ygd[2 * o:], xgd[2 * o:], cnt, d, stream)
+
+    def persistent():
+        b.execute(yd, xd, stream)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    t_p = timed(persistent)
+    t_s = timed(per_size)
+    res = {"workload": "20000 complex-f32 transforms, lengths uniform over the 328 7-smooth N in [18,8192] "
+                       "and primes in [17,127], direction Bernoulli(1/2), data U[-1,1) complex (seed 3)",
+           "points": total, "ms_persistent": round(t_p, 4), "ms_per_size": round(t_s, 4),
+           "launches_per_size": len(runs), "speedup": round(t_s / t_p, 2),
+           "gbs_persistent": round(16.0 * total / (t_p * 1e-3) * 1e-9, 1),
+           "frac_persistent": round(16.0 * total / (t_p * 1e-3) * 1e-9 / HBM_PEAK_GBS, 4),
+           "plan_s": round(plan_s, 3), "schedule": b.describe()}
+    if check:
+        y = yd.cpu().numpy().view(np.complex64)
+        worst = 0.0
+        for i in rng.choice(sizes.size, 200, replace=False):
+            n, o = int(sizes[i]), int(offs[i])
+            xx = x[o:o + n].astype(np.complex128)
+            ref = np.fft.ifft(xx) * n if inv[i] else np.fft.fft(xx)
+            worst = max(worst, float(np.linalg.norm(y[o:o + n] - ref) / np.linalg.norm(ref)))
+        res["rel_rms_vs_fp64_max_sampled"] = float(f"{worst:.3e}")
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +161,7 @@ def main() -> int:
     ap.add_argument("--quick", action="store_true", help="sizes up to 2^20 only, C2C only")
     ap.add_argument("--only", default=None,
                     help="comma-separated subset of configurations kind:dtype:n (e.g. c2c:f32:4096), for profiling")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the mixed-length batch measurement")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one plain 256 MiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -224,6 +305,10 @@ def main() -> int:
     fs_t = sum(p["ms"] for p in fs)
     fs_bytes = sum(alg_bytes("c2c", "f32", p["n"], p["batch"]) for p in fs)
 
+    uneven = None
+    if not args.no_uneven and not args.only:
+        uneven = uneven_batch(torch, kfr_amd, stream, max(3, args.steps), check=True)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(cfgs)
@@ -247,6 +332,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "rel_rms_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs.items()},
+        "uneven_batch": uneven,
         "fourstep_c2c_f32_frac": round(fs_bytes / (fs_t * 1e-3) * 1e-9 / HBM_PEAK_GBS, 4) if fs_t else None,
         "wall_s": round(wall, 3),
         "per_config": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in p.items()} for p in per],

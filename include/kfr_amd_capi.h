@@ -227,6 +227,23 @@ KFR_AMD_API int kfr_amd_conv_backward(KFR_AMD_CONV* conv, void* dx, void* dh, co
                                       const void* h, size_t h_dist, size_t batch, void* stream);
 KFR_AMD_API void kfr_amd_conv_delete(KFR_AMD_CONV* conv);
 
+/* Uneven batches: `count` complex transforms of lengths sizes[i] and directions inverse[i] (NULL: all
+ * forward) in one launch, as one persistent kernel fed from a device work queue and replayed as a
+ * hipGraph. Transform i reads and writes sizes[0] + ... + sizes[i-1] complex values into the buffers.
+ * Every length must fit one workgroup (f32 up to 8192 points, f64 up to 4096; lengths with a prime
+ * factor of 101 or more run Bluestein inside the same kernel and may be up to half that). autotune != 0
+ * times the candidate transforms-per-workgroup of each length on the device at creation (cached per
+ * process). execute: device pointers are enqueued on `stream` (NULL = default stream) without
+ * synchronising; host pointers are staged synchronously. Calls on one batch plan must not overlap.
+ * dtype: 0 f32, 1 f64. No counterpart in the reference, which transforms one length per plan. */
+typedef struct KFR_AMD_BATCH { int opaque; } KFR_AMD_BATCH;
+KFR_AMD_API KFR_AMD_BATCH* kfr_amd_batch_create(int dtype, size_t count, const size_t* sizes, const int* inverse,
+                                                int autotune);
+KFR_AMD_API size_t kfr_amd_batch_total_size(KFR_AMD_BATCH* batch);
+KFR_AMD_API int kfr_amd_batch_execute(KFR_AMD_BATCH* batch, void* out, const void* in, void* stream);
+KFR_AMD_API int kfr_amd_batch_describe(KFR_AMD_BATCH* batch, char* buf, size_t buflen);
+KFR_AMD_API void kfr_amd_batch_delete(KFR_AMD_BATCH* batch);
+
 /* The reference plan descriptor for a size, computed on the host without a GPU: text in the layout of
  * dft_plan::dump() (one line per stage) preceded by a summary line. kind: 0 dft_plan, 1 dft_plan_real
  * (arg = pack format), 2 dct_plan, 3 dft_plan with progressive_optimized. Returns the text length

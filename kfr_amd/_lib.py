@@ -86,6 +86,11 @@ def _proto(lib):
     f("kfr_amd_dft_permutation", C.c_int, C.c_int, c_size, vp)
     f("kfr_amd_dft_progressive_total_steps", c_size, C.c_int, c_size)
     f("kfr_amd_dft_schedule", C.c_int, C.c_int, c_size, C.c_char_p, c_size)
+    f("kfr_amd_batch_create", vp, C.c_int, c_size, vp, vp, C.c_int)
+    f("kfr_amd_batch_total_size", c_size, vp)
+    f("kfr_amd_batch_execute", C.c_int, vp, vp, vp, vp)
+    f("kfr_amd_batch_describe", C.c_int, vp, C.c_char_p, c_size)
+    f("kfr_amd_batch_delete", None, vp)
     f("kfr_amd_inject_alloc_failure", None, C.c_int)
     f("kfr_amd_synchronize", C.c_int)
 

@@ -1,6 +1,7 @@
 // C ABI of the gfx950 DFT engine: the reference's kfr_dft_* / kfr_dft_real_* / kfr_dct_* /
 // kfr_filter_* entry points (src/capi/capi.cpp) over GPU plans, plus the device extensions.
 #include "../../include/kfr_amd_capi.h"
+#include "fft_batch.hpp"
 #include "fft_engine.hpp"
 #include "plan_desc.hpp"
 
@@ -270,6 +271,23 @@ struct conv_obj
     std::unique_ptr<gpu_conv<float>> f;
     std::unique_ptr<gpu_conv<double>> d;
     std::mutex mu;
+};
+
+struct batch_obj
+{
+    std::unique_ptr<gpu_batch<float>> f;
+    std::unique_ptr<gpu_batch<double>> d;
+    dev_mem buf; // staging for host pointers
+    std::mutex mu;
+    size_t total() const { return f ? f->total() : d->total(); }
+    size_t elem() const { return f ? sizeof(float) * 2 : sizeof(double) * 2; }
+    void run(void* out, const void* in, hipStream_t s)
+    {
+        if (f)
+            f->execute(out, in, s);
+        else
+            d->execute(out, in, s);
+    }
 };
 
 int copy_text(const std::string& s, char* buf, size_t buflen)
@@ -875,6 +893,83 @@ extern "C"
                 return copy_text(gpu_dft<double>(size).schedule().text(), buf, buflen);
             },
             -1);
+    }
+
+    KFR_AMD_BATCH* kfr_amd_batch_create(int dtype, size_t count, const size_t* sizes, const int* inverse,
+                                        int autotune)
+    {
+        return try_fn(
+            [&]
+            {
+                const dtype_t dt = to_dtype(dtype);
+                if (count && !sizes)
+                    throw std::invalid_argument("sizes is NULL");
+                auto* b = new batch_obj();
+                try
+                {
+                    if (dt == dtype_t::f32)
+                        b->f.reset(new gpu_batch<float>(count, sizes, inverse, autotune != 0));
+                    else
+                        b->d.reset(new gpu_batch<double>(count, sizes, inverse, autotune != 0));
+                }
+                catch (...)
+                {
+                    delete b;
+                    throw;
+                }
+                return reinterpret_cast<KFR_AMD_BATCH*>(b);
+            },
+            (KFR_AMD_BATCH*)nullptr);
+    }
+    size_t kfr_amd_batch_total_size(KFR_AMD_BATCH* batch)
+    {
+        return try_fn(
+            [&]
+            {
+                check_plan(batch);
+                return reinterpret_cast<batch_obj*>(batch)->total();
+            },
+            (size_t)0);
+    }
+    int kfr_amd_batch_execute(KFR_AMD_BATCH* batch, void* out, const void* in, void* stream)
+    {
+        return try_fn(
+            [&]
+            {
+                check_plan(batch);
+                auto* b = reinterpret_cast<batch_obj*>(batch);
+                std::lock_guard<std::mutex> lock(b->mu);
+                if (is_device_ptr(in) && is_device_ptr(out))
+                {
+                    b->run(out, in, as_stream(stream));
+                    return 0;
+                }
+                // host data: staged through one device buffer, synchronously
+                const size_t bytes = b->total() * b->elem();
+                hipStream_t s      = library_stream();
+                b->buf.ensure(bytes);
+                hip_check(hipMemcpyAsync(b->buf.ptr, in, bytes, hipMemcpyDefault, s), "batch input copy");
+                b->run(b->buf.ptr, b->buf.ptr, s);
+                hip_check(hipMemcpyAsync(out, b->buf.ptr, bytes, hipMemcpyDefault, s), "batch output copy");
+                hip_check(hipStreamSynchronize(s), "batch execute");
+                return 0;
+            },
+            -1);
+    }
+    int kfr_amd_batch_describe(KFR_AMD_BATCH* batch, char* buf, size_t buflen)
+    {
+        return try_fn(
+            [&]
+            {
+                check_plan(batch);
+                auto* b = reinterpret_cast<batch_obj*>(batch);
+                return copy_text(b->f ? b->f->describe() : b->d->describe(), buf, buflen);
+            },
+            -1);
+    }
+    void kfr_amd_batch_delete(KFR_AMD_BATCH* batch)
+    {
+        try_void([&] { delete reinterpret_cast<batch_obj*>(batch); });
     }
 
     void kfr_amd_inject_alloc_failure(int n) { set_alloc_failure_countdown(n); }

@@ -320,10 +320,10 @@ struct stockham
 
 inline bool lds_radix(int r) { return r == 2 || r == 3 || r == 4 || r == 5 || r == 7 || r == 8 || r == 16; }
 
-// Factor n into GPU passes. Power-of-2 part: the largest butterfly that fits a thread's pm values
+// Factor n into GPU passes (wide: the ceil(pm/R) butterflies per thread of radix_pass<..., WIDE>). Power-of-2 part: the largest butterfly that fits a thread's pm values
 // (16 for f32, 8 for f64) first, remainder 8/4/2; then 7, 5, 3; any other prime factor up to 127
 // becomes a direct DFT pass, placed last.
-inline bool make_stockham(long long nn, stockham& S, int pm)
+inline bool make_stockham(long long nn, stockham& S, int pm, bool wide = false)
 {
     S = stockham{};
     if (nn < 1 || nn > (1 << 28))
@@ -389,7 +389,7 @@ inline bool make_stockham(long long nn, stockham& S, int pm)
         {
             if (ns > 1)
                 off += ns * (r - 1);
-            const int M  = std::max(1, pm / r);
+            const int M  = wide ? (pm + r - 1) / r : std::max(1, pm / r); // bflies_of
             const int nb = n / r;
             tpt          = std::max(tpt, (nb + M - 1) / M);
         }
@@ -702,6 +702,12 @@ __global__ void __launch_bounds__(max_block_of(MIXED)) k_fft(lds_geom G, const c
         }
     }
 }
+
+// Bluestein tables of an n-point transform padded to M points: chirp[m] = exp(-i pi (m^2 mod 2n)/n)
+// (n values) and H = FFT_M(h)/M with h the chirp mirrored into both ends of M points (M values), both
+// computed in FP64 and rounded once. Synchronises s.
+template <typename T>
+void bluestein_tables(size_t n, size_t M, cx<T>* chirp, cx<T>* H, hipStream_t s);
 
 // ------------------------------------------------------------------------------------------------
 // A transform computed without Bluestein: one LDS kernel, or four-step in two kernels.

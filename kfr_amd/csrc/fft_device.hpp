@@ -370,12 +370,17 @@ struct fft_ctx
     __device__ __forceinline__ cx<T>& at(int i) const { return lds[lds_pad(base + i * ls, ps)]; }
 };
 
+// butterflies per thread in a radix-R pass: floor(pmax/R) (at least 1), or with WIDE ceil(pmax/R), so
+// that R*M >= pmax and every length up to 512*pmax runs on at most 512 threads per transform
+template <typename T, int R, bool WIDE>
+inline constexpr int bflies_of = WIDE ? (pmax_of<T> + R - 1) / R : (pmax_of<T> / R > 0 ? pmax_of<T> / R : 1);
+
 // one radix pass, LDS -> LDS
-template <typename T, int R, bool INV>
+template <typename T, int R, bool INV, bool WIDE = false>
 __device__ __forceinline__ void radix_pass(const lds_geom& G, const pass_desc& P, const cx<T>* __restrict__ tw,
                                            const fft_ctx<T>& c)
 {
-    constexpr int M = pmax_of<T> / R > 0 ? pmax_of<T> / R : 1;
+    constexpr int M = bflies_of<T, R, WIDE>;
     const int nb    = G.n / R;
     cx<T> v[M][R];
     int kk[M];
@@ -474,8 +479,10 @@ __device__ __forceinline__ void generic_pass(const lds_geom& G, const pass_desc&
 }
 
 // all passes of the transform held in LDS (natural order in, natural order out). MIXED = false
-// compiles only the power-of-two butterflies (radix 16 for f32, 8 for f64, and the remainders).
-template <typename T, bool INV, bool MIXED>
+// compiles only the power-of-two butterflies (radix 16 for f32, 8 for f64, and the remainders). WIDE
+// selects the ceil(pmax/R) butterflies per thread of radix_pass (the geometry must come from
+// make_stockham with wide = true).
+template <typename T, bool INV, bool MIXED, bool WIDE = false>
 __device__ __forceinline__ void fft_passes(const lds_geom& G, const cx<T>* tw, const fft_ctx<T>& c)
 {
     for (int p = 0; p < G.np; ++p)
@@ -484,17 +491,17 @@ __device__ __forceinline__ void fft_passes(const lds_geom& G, const cx<T>* tw, c
         switch (P.radix)
         {
         case 2:
-            radix_pass<T, 2, INV>(G, P, tw, c);
+            radix_pass<T, 2, INV, WIDE>(G, P, tw, c);
             break;
         case 4:
-            radix_pass<T, 4, INV>(G, P, tw, c);
+            radix_pass<T, 4, INV, WIDE>(G, P, tw, c);
             break;
         case 8:
-            radix_pass<T, 8, INV>(G, P, tw, c);
+            radix_pass<T, 8, INV, WIDE>(G, P, tw, c);
             break;
         case 16:
             if constexpr (sizeof(T) == 4)
-                radix_pass<T, 16, INV>(G, P, tw, c);
+                radix_pass<T, 16, INV, WIDE>(G, P, tw, c);
             break;
         default:
             if constexpr (MIXED)
@@ -502,13 +509,13 @@ __device__ __forceinline__ void fft_passes(const lds_geom& G, const cx<T>* tw, c
                 switch (P.radix)
                 {
                 case 3:
-                    radix_pass<T, 3, INV>(G, P, tw, c);
+                    radix_pass<T, 3, INV, WIDE>(G, P, tw, c);
                     break;
                 case 5:
-                    radix_pass<T, 5, INV>(G, P, tw, c);
+                    radix_pass<T, 5, INV, WIDE>(G, P, tw, c);
                     break;
                 case 7:
-                    radix_pass<T, 7, INV>(G, P, tw, c);
+                    radix_pass<T, 7, INV, WIDE>(G, P, tw, c);
                     break;
                 default:
                     generic_pass<T, INV>(G, P, tw, c);

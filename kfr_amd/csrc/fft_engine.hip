@@ -126,6 +126,32 @@ __global__ void k_convert_scale(cx<T>* out, const cx<double>* in, long long coun
         out[i] = { (T)(in[i].re * scale), (T)(in[i].im * scale) };
 }
 
+template <typename T>
+void bluestein_tables(size_t n, size_t M, cx<T>* chirp, cx<T>* H, hipStream_t s)
+{
+    dev_mem c64(n * sizeof(cx<double>));
+    gen_twiddles(c64.as<cx<double>>(), (long long)n, 2 * (long long)n, 0, 0, tw_square, 0, s);
+    hipLaunchKernelGGL(k_convert_scale<T>, dim3(nblocks(n, 256)), dim3(256), 0, s, chirp, c64.as<cx<double>>(),
+                       (long long)n, 1.0);
+    hip_check(hipGetLastError(), "chirp conversion");
+    // H = FFT_M(h) / M evaluated in FP64 and rounded once
+    dev_mem h64(M * sizeof(cx<double>));
+    hipLaunchKernelGGL(k_bluestein_h, dim3(nblocks(M, 256)), dim3(256), 0, s, h64.as<cx<double>>(),
+                       c64.as<cx<double>>(), (long long)n, (long long)M);
+    hip_check(hipGetLastError(), "bluestein kernel spectrum");
+    {
+        core_fft<double> f64(M, s);
+        f64.run<false>(load_plain<double>(h64.ptr, 0), store_plain<double>(h64.ptr, 0), 1, s);
+        hip_check(hipStreamSynchronize(s), "bluestein kernel spectrum");
+    }
+    hipLaunchKernelGGL(k_convert_scale<T>, dim3(nblocks(M, 256)), dim3(256), 0, s, H, h64.as<cx<double>>(),
+                       (long long)M, 1.0 / (double)M);
+    hip_check(hipGetLastError(), "bluestein kernel spectrum");
+    hip_check(hipStreamSynchronize(s), "bluestein tables"); // before the FP64 buffers are released
+}
+template void bluestein_tables<float>(size_t, size_t, cx<float>*, cx<float>*, hipStream_t);
+template void bluestein_tables<double>(size_t, size_t, cx<double>*, cx<double>*, hipStream_t);
+
 // ------------------------------------------------------------------------------------------------
 // complex plan
 
@@ -178,26 +204,9 @@ struct gpu_dft<T>::impl
     {
         M                = next_pow2(2 * n - 1);
         info.bluestein_m = M;
-        dev_mem c64(n * sizeof(cx<double>));
-        gen_twiddles(c64.as<cx<double>>(), (long long)n, 2 * (long long)n, 0, 0, tw_square, 0, s);
         chirp.reset(n * sizeof(cx<T>));
-        hipLaunchKernelGGL(k_convert_scale<T>, dim3(nblocks(n, 256)), dim3(256), 0, s, chirp.as<cx<T>>(),
-                           c64.as<cx<double>>(), (long long)n, 1.0);
-        hip_check(hipGetLastError(), "chirp conversion");
-        // H = FFT_M(h) / M evaluated in FP64 and rounded once
-        dev_mem h64(M * sizeof(cx<double>));
-        hipLaunchKernelGGL(k_bluestein_h, dim3(nblocks(M, 256)), dim3(256), 0, s, h64.as<cx<double>>(),
-                           c64.as<cx<double>>(), (long long)n, (long long)M);
-        hip_check(hipGetLastError(), "bluestein kernel spectrum");
-        {
-            core_fft<double> f64(M, s);
-            f64.run<false>(load_plain<double>(h64.ptr, 0), store_plain<double>(h64.ptr, 0), 1, s);
-            hip_check(hipStreamSynchronize(s), "bluestein kernel spectrum");
-        }
         H.reset(M * sizeof(cx<T>));
-        hipLaunchKernelGGL(k_convert_scale<T>, dim3(nblocks(M, 256)), dim3(256), 0, s, H.as<cx<T>>(),
-                           h64.as<cx<double>>(), (long long)M, 1.0 / (double)M);
-        hip_check(hipGetLastError(), "bluestein kernel spectrum");
+        bluestein_tables<T>(n, M, chirp.as<cx<T>>(), H.as<cx<T>>(), s);
         if (core_fft<T>::fits_lds(M))
         {
             algo = BS_FUSED;

@@ -276,6 +276,51 @@ class convolve_filter:
             self._h = None
 
 
+class dft_batch:
+    """Many complex transforms of uneven lengths and per-transform directions in one launch
+    (kfr_amd_batch_*). Transform i reads and writes sizes[i] values starting at the sum of the sizes
+    before it. No counterpart in the reference, which plans one length at a time."""
+
+    def __init__(self, sizes, inverse=None, dtype=np.complex64, autotune: bool = False):
+        self.dtype = np.dtype(dtype)
+        self.sizes = np.ascontiguousarray(sizes, dtype=np.uint64)
+        inv = None if inverse is None else np.ascontiguousarray(inverse, dtype=np.int32)
+        if inv is not None and inv.size != self.sizes.size:
+            raise ValueError("inverse must have one flag per transform")
+        self._inv = inv
+        self._p = lib().kfr_amd_batch_create(0 if _tag(dtype) == "f32" else 1, self.sizes.size,
+                                            self.sizes.ctypes.data, None if inv is None else inv.ctypes.data,
+                                            1 if autotune else 0)
+        check(bool(self._p))
+
+    @property
+    def total_size(self) -> int:
+        return int(lib().kfr_amd_batch_total_size(self._p))
+
+    def offsets(self) -> np.ndarray:
+        return np.concatenate([[0], np.cumsum(self.sizes.astype(np.int64))[:-1]])
+
+    def execute(self, out, in_, stream=None):
+        """Device tensors are enqueued on the stream (torch's current stream by default); numpy arrays
+        are staged synchronously."""
+        if _nbytes(out) < self.total_size * self.dtype.itemsize or _nbytes(in_) < self.total_size * self.dtype.itemsize:
+            raise ValueError("buffers smaller than the batch")
+        s = _stream_ptr(stream) if not isinstance(out, np.ndarray) else None
+        check(lib().kfr_amd_batch_execute(self._p, _ptr(out), _ptr(in_), s) == 0)
+        return out
+
+    def describe(self) -> str:
+        buf = C.create_string_buffer(4096)
+        check(lib().kfr_amd_batch_describe(self._p, buf, len(buf)) >= 0)
+        return buf.value.decode()
+
+    def __del__(self):
+        p = getattr(self, "_p", None)
+        if p:
+            lib().kfr_amd_batch_delete(p)
+            self._p = None
+
+
 class dft_cache:
     """Process-wide cache of complex plans (include/kfr/dft/cache.hpp:44-126). The key includes the
     element type and the size, as the reference's does."""

@@ -48,3 +48,18 @@ def fp64_dct(x: np.ndarray, inverse: bool = False) -> np.ndarray:
     import scipy.fft
     # scipy's unnormalised DCT-II is 2 sum x cos(...); DCT-III is x0 + 2 sum_{k>0} x_k cos(...)
     return scipy.fft.dct(np.asarray(x, dtype=np.float64), type=3 if inverse else 2, axis=-1) / 2.0
+
+
+def uneven_lengths() -> np.ndarray:
+    """Lengths of the mixed-length batch workload: every 7-smooth N in [18, 8192] (2^a 3^b 5^c 7^d)
+    and every prime in [17, 127]."""
+    smooth = set()
+    for a in range(14):
+        for b in range(9):
+            for c in range(6):
+                for d in range(5):
+                    v = 2 ** a * 3 ** b * 5 ** c * 7 ** d
+                    if 18 <= v <= 8192:
+                        smooth.add(v)
+    primes = [p for p in range(17, 128) if all(p % q for q in range(2, int(p ** 0.5) + 1))]
+    return np.array(sorted(smooth | set(primes)), dtype=np.int64)
