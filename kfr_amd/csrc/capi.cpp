@@ -3,6 +3,7 @@
 #include "../../include/kfr_amd_capi.h"
 #include "fft_batch.hpp"
 #include "fft_engine.hpp"
+#include "fft_md.hpp"
 #include "plan_desc.hpp"
 
 #include <hip/hip_runtime.h>
@@ -111,26 +112,115 @@ struct stager
     }
 };
 
+// a complex plan: one axis (dft_plan) or several (dft_plan_md), as the reference's C API keeps both
+// behind the same handle (src/capi/capi.cpp:96-173)
 template <typename T>
 struct dft_obj
 {
-    plan_desc desc;
-    gpu_dft<T> g;
+    std::vector<plan_desc> desc;       // the reference plan of every axis
+    std::unique_ptr<gpu_dft<T>> g;     // one axis
+    std::unique_ptr<gpu_dft_md<T>> md; // two or more axes
     stager st;
-    explicit dft_obj(size_t n)
-        : desc(describe_dft(sizeof(T) == 4 ? dtype_t::f32 : dtype_t::f64, n, false)), g(n)
+    explicit dft_obj(const std::vector<size_t>& shape)
     {
+        const dtype_t dt = sizeof(T) == 4 ? dtype_t::f32 : dtype_t::f64;
+        if (shape.empty())
+            throw std::invalid_argument("a DFT plan needs at least one axis");
+        for (size_t n : shape)
+            desc.push_back(describe_dft(dt, n, false));
+        if (shape.size() == 1)
+            g.reset(new gpu_dft<T>(shape[0]));
+        else
+            md.reset(new gpu_dft_md<T>(shape));
+    }
+    size_t size() const { return g ? g->size() : md->size(); }
+    size_t temp_size() const
+    {
+        size_t t = 0;
+        for (const auto& d : desc)
+            t = std::max(t, d.temp_size);
+        return t;
+    }
+    std::string schedule() const { return g ? g->schedule().text() : md->schedule(); }
+    void execute(void* out, const void* in, long long batch, bool inverse, hipStream_t s)
+    {
+        if (g)
+        {
+            g->execute(out, in, batch, inverse, s);
+            return;
+        }
+        const size_t bytes = md->size() * 2 * sizeof(T);
+        for (long long b = 0; b < batch; ++b)
+            md->execute(static_cast<char*>(out) + b * bytes, static_cast<const char*>(in) + b * bytes, inverse, s);
     }
 };
+
+// a real plan: one axis (dft_plan_real) or several (dft_plan_md_real, CCs along the last axis)
 template <typename T>
 struct real_obj
 {
-    plan_desc desc;
-    gpu_dft_real<T> g;
+    std::vector<plan_desc> desc;            // complex axes first, then the real last axis
+    std::unique_ptr<gpu_dft_real<T>> g;     // one axis
+    std::unique_ptr<gpu_dft_md_real<T>> md; // two or more axes
+    size_t extra_temp = 0;                  // md plans without real_out_is_enough keep the complex array
     stager st;
     real_obj(size_t n, int fmt)
-        : desc(describe_dft_real(sizeof(T) == 4 ? dtype_t::f32 : dtype_t::f64, n, fmt)), g(n, fmt)
     {
+        desc.push_back(describe_dft_real(sizeof(T) == 4 ? dtype_t::f32 : dtype_t::f64, n, fmt));
+        g.reset(new gpu_dft_real<T>(n, fmt));
+    }
+    // dft_plan_md_real(shape, real_out_is_enough), include/kfr/dft/fft.hpp:739-762
+    real_obj(const std::vector<size_t>& shape, bool real_out_is_enough)
+    {
+        const dtype_t dt = sizeof(T) == 4 ? dtype_t::f32 : dtype_t::f64;
+        if (shape.empty())
+            throw std::invalid_argument("a DFT plan needs at least one axis");
+        for (size_t i = 0; i + 1 < shape.size(); ++i)
+            desc.push_back(describe_dft(dt, shape[i], false));
+        desc.push_back(describe_dft_real(dt, shape.back(), 1));
+        if (shape.size() == 1)
+            g.reset(new gpu_dft_real<T>(shape[0], 1));
+        else
+            md.reset(new gpu_dft_md_real<T>(shape));
+        if (!real_out_is_enough)
+            extra_temp = complex_size() * 2 * sizeof(T);
+    }
+    size_t size() const { return g ? g->size() : md->size(); }
+    size_t complex_size() const
+    {
+        if (g)
+            return g->complex_size();
+        return md->complex_size();
+    }
+    size_t temp_size() const
+    {
+        size_t t = 0;
+        for (const auto& d : desc)
+            t = std::max(t, d.temp_size);
+        return t + extra_temp;
+    }
+    std::string schedule() const { return g ? g->schedule().text() : md->schedule(); }
+    void forward(void* out, const void* in, long long batch, hipStream_t s)
+    {
+        if (g)
+        {
+            g->forward(out, in, batch, s);
+            return;
+        }
+        for (long long b = 0; b < batch; ++b)
+            md->forward(static_cast<char*>(out) + b * complex_size() * 2 * sizeof(T),
+                        static_cast<const char*>(in) + b * size() * sizeof(T), s);
+    }
+    void inverse(void* out, const void* in, long long batch, hipStream_t s)
+    {
+        if (g)
+        {
+            g->inverse(out, in, batch, s);
+            return;
+        }
+        for (long long b = 0; b < batch; ++b)
+            md->inverse(static_cast<char*>(out) + b * size() * sizeof(T),
+                        static_cast<const char*>(in) + b * complex_size() * 2 * sizeof(T), s);
     }
 };
 template <typename T>
@@ -150,10 +240,11 @@ struct filter_obj
     filter_obj(const T* taps, size_t n, size_t block) : g(taps, n, block) {}
 };
 
-void print_dump(const plan_desc& d, const schedule_info& s)
+void print_dump(const std::vector<plan_desc>& ds, const std::string& sched)
 {
-    std::fputs(dump_text(d, "gfx950").c_str(), stdout);
-    std::printf("%s\n", s.text().c_str());
+    for (const auto& d : ds)
+        std::fputs(dump_text(d, "gfx950").c_str(), stdout);
+    std::printf("%s\n", sched.c_str());
     std::fflush(stdout);
 }
 
@@ -180,19 +271,19 @@ template <typename T>
 void dft_exec(void* plan, void* out, const void* in, bool inverse)
 {
     auto* o        = dft_cast<T>(plan);
-    const size_t b = o->g.size() * 2 * sizeof(T);
-    o->st.run(out, b, in, b, [&](void* dout, const void* din, hipStream_t s) { o->g.execute(dout, din, 1, inverse, s); });
+    const size_t b = o->size() * 2 * sizeof(T);
+    o->st.run(out, b, in, b, [&](void* dout, const void* din, hipStream_t s) { o->execute(dout, din, 1, inverse, s); });
 }
 template <typename T>
 void real_exec(void* plan, void* out, const void* in, bool inverse)
 {
     auto* o         = real_cast<T>(plan);
-    const size_t rb = o->g.size() * sizeof(T);
-    const size_t cb = o->g.complex_size() * 2 * sizeof(T);
+    const size_t rb = o->size() * sizeof(T);
+    const size_t cb = o->complex_size() * 2 * sizeof(T);
     if (!inverse)
-        o->st.run(out, cb, in, rb, [&](void* dout, const void* din, hipStream_t s) { o->g.forward(dout, din, 1, s); });
+        o->st.run(out, cb, in, rb, [&](void* dout, const void* din, hipStream_t s) { o->forward(dout, din, 1, s); });
     else
-        o->st.run(out, rb, in, cb, [&](void* dout, const void* din, hipStream_t s) { o->g.inverse(dout, din, 1, s); });
+        o->st.run(out, rb, in, cb, [&](void* dout, const void* din, hipStream_t s) { o->inverse(dout, din, 1, s); });
 }
 template <typename T>
 void dct_exec(void* plan, void* out, const void* in, bool inverse)
@@ -223,6 +314,14 @@ void filter_process(void* plan, T* out, const T* in, size_t n)
     f->g.process(f->out_buf.template as<T>(), f->in_buf.template as<T>(), n, s);
     hip_check(hipMemcpyAsync(out, f->out_buf.ptr, n * sizeof(T), hipMemcpyDefault, s), "filter output copy");
     hip_check(hipStreamSynchronize(s), "filter process");
+}
+
+// init_shape (src/capi/capi.cpp:105-113)
+std::vector<size_t> shape_of(size_t dims, const unsigned* shape)
+{
+    if (dims == 0 || !shape)
+        throw std::invalid_argument("shape is empty");
+    return std::vector<size_t>(shape, shape + dims);
 }
 
 [[noreturn]] void not_provided(const char* what)
@@ -341,61 +440,69 @@ extern "C"
     // ---- complex DFT
     KFR_DFT_PLAN_F32* kfr_dft_create_plan_f32(size_t size)
     {
-        return try_fn([&] { return reinterpret_cast<KFR_DFT_PLAN_F32*>(new dft_obj<float>(size)); },
+        return try_fn([&] { return reinterpret_cast<KFR_DFT_PLAN_F32*>(new dft_obj<float>(std::vector<size_t>{ size })); },
                       (KFR_DFT_PLAN_F32*)nullptr);
     }
     KFR_DFT_PLAN_F64* kfr_dft_create_plan_f64(size_t size)
     {
-        return try_fn([&] { return reinterpret_cast<KFR_DFT_PLAN_F64*>(new dft_obj<double>(size)); },
+        return try_fn([&] { return reinterpret_cast<KFR_DFT_PLAN_F64*>(new dft_obj<double>(std::vector<size_t>{ size })); },
                       (KFR_DFT_PLAN_F64*)nullptr);
     }
-    KFR_DFT_PLAN_F32* kfr_dft_create_2d_plan_f32(size_t, size_t)
+    KFR_DFT_PLAN_F32* kfr_dft_create_2d_plan_f32(size_t size1, size_t size2)
     {
-        return try_fn([&]() -> KFR_DFT_PLAN_F32* { not_provided("multidimensional DFT"); }, (KFR_DFT_PLAN_F32*)nullptr);
+        return try_fn([&] { return reinterpret_cast<KFR_DFT_PLAN_F32*>(new dft_obj<float>(std::vector<size_t>{ size1, size2 })); },
+                      (KFR_DFT_PLAN_F32*)nullptr);
     }
-    KFR_DFT_PLAN_F64* kfr_dft_create_2d_plan_f64(size_t, size_t)
+    KFR_DFT_PLAN_F64* kfr_dft_create_2d_plan_f64(size_t size1, size_t size2)
     {
-        return try_fn([&]() -> KFR_DFT_PLAN_F64* { not_provided("multidimensional DFT"); }, (KFR_DFT_PLAN_F64*)nullptr);
+        return try_fn([&] { return reinterpret_cast<KFR_DFT_PLAN_F64*>(new dft_obj<double>(std::vector<size_t>{ size1, size2 })); },
+                      (KFR_DFT_PLAN_F64*)nullptr);
     }
-    KFR_DFT_PLAN_F32* kfr_dft_create_3d_plan_f32(size_t, size_t, size_t)
+    KFR_DFT_PLAN_F32* kfr_dft_create_3d_plan_f32(size_t size1, size_t size2, size_t size3)
     {
-        return try_fn([&]() -> KFR_DFT_PLAN_F32* { not_provided("multidimensional DFT"); }, (KFR_DFT_PLAN_F32*)nullptr);
+        return try_fn(
+            [&] { return reinterpret_cast<KFR_DFT_PLAN_F32*>(new dft_obj<float>(std::vector<size_t>{ size1, size2, size3 })); },
+            (KFR_DFT_PLAN_F32*)nullptr);
     }
-    KFR_DFT_PLAN_F64* kfr_dft_create_3d_plan_f64(size_t, size_t, size_t)
+    KFR_DFT_PLAN_F64* kfr_dft_create_3d_plan_f64(size_t size1, size_t size2, size_t size3)
     {
-        return try_fn([&]() -> KFR_DFT_PLAN_F64* { not_provided("multidimensional DFT"); }, (KFR_DFT_PLAN_F64*)nullptr);
+        return try_fn(
+            [&] { return reinterpret_cast<KFR_DFT_PLAN_F64*>(new dft_obj<double>(std::vector<size_t>{ size1, size2, size3 })); },
+            (KFR_DFT_PLAN_F64*)nullptr);
     }
-    KFR_DFT_PLAN_F32* kfr_dft_create_md_plan_f32(size_t, const unsigned*)
+    KFR_DFT_PLAN_F32* kfr_dft_create_md_plan_f32(size_t dims, const unsigned* shape)
     {
-        return try_fn([&]() -> KFR_DFT_PLAN_F32* { not_provided("multidimensional DFT"); }, (KFR_DFT_PLAN_F32*)nullptr);
+        return try_fn([&] { return reinterpret_cast<KFR_DFT_PLAN_F32*>(new dft_obj<float>(shape_of(dims, shape))); },
+                      (KFR_DFT_PLAN_F32*)nullptr);
     }
-    KFR_DFT_PLAN_F64* kfr_dft_create_md_plan_f64(size_t, const unsigned*)
+    KFR_DFT_PLAN_F64* kfr_dft_create_md_plan_f64(size_t dims, const unsigned* shape)
     {
-        return try_fn([&]() -> KFR_DFT_PLAN_F64* { not_provided("multidimensional DFT"); }, (KFR_DFT_PLAN_F64*)nullptr);
+        return try_fn([&] { return reinterpret_cast<KFR_DFT_PLAN_F64*>(new dft_obj<double>(shape_of(dims, shape))); },
+                      (KFR_DFT_PLAN_F64*)nullptr);
     }
     void kfr_dft_dump_f32(KFR_DFT_PLAN_F32* plan)
     {
-        try_void([&] { print_dump(dft_cast<float>(plan)->desc, dft_cast<float>(plan)->g.schedule()); });
+        try_void([&] { print_dump(dft_cast<float>(plan)->desc, dft_cast<float>(plan)->schedule()); });
     }
     void kfr_dft_dump_f64(KFR_DFT_PLAN_F64* plan)
     {
-        try_void([&] { print_dump(dft_cast<double>(plan)->desc, dft_cast<double>(plan)->g.schedule()); });
+        try_void([&] { print_dump(dft_cast<double>(plan)->desc, dft_cast<double>(plan)->schedule()); });
     }
     size_t kfr_dft_get_size_f32(KFR_DFT_PLAN_F32* plan)
     {
-        return try_fn([&] { return dft_cast<float>(plan)->g.size(); }, (size_t)0);
+        return try_fn([&] { return dft_cast<float>(plan)->size(); }, (size_t)0);
     }
     size_t kfr_dft_get_size_f64(KFR_DFT_PLAN_F64* plan)
     {
-        return try_fn([&] { return dft_cast<double>(plan)->g.size(); }, (size_t)0);
+        return try_fn([&] { return dft_cast<double>(plan)->size(); }, (size_t)0);
     }
     size_t kfr_dft_get_temp_size_f32(KFR_DFT_PLAN_F32* plan)
     {
-        return try_fn([&] { return dft_cast<float>(plan)->desc.temp_size; }, (size_t)0);
+        return try_fn([&] { return dft_cast<float>(plan)->temp_size(); }, (size_t)0);
     }
     size_t kfr_dft_get_temp_size_f64(KFR_DFT_PLAN_F64* plan)
     {
-        return try_fn([&] { return dft_cast<double>(plan)->desc.temp_size; }, (size_t)0);
+        return try_fn([&] { return dft_cast<double>(plan)->temp_size(); }, (size_t)0);
     }
     void kfr_dft_execute_f32(KFR_DFT_PLAN_F32* plan, kfr_c32* out, const kfr_c32* in, uint8_t*)
     {
@@ -433,59 +540,77 @@ extern "C"
         return try_fn([&] { return reinterpret_cast<KFR_DFT_REAL_PLAN_F64*>(new real_obj<double>(size, (int)fmt)); },
                       (KFR_DFT_REAL_PLAN_F64*)nullptr);
     }
-    KFR_DFT_REAL_PLAN_F32* kfr_dft_real_create_2d_plan_f32(size_t, size_t, kfr_bool)
+    KFR_DFT_REAL_PLAN_F32* kfr_dft_real_create_2d_plan_f32(size_t size1, size_t size2, kfr_bool real_out_is_enough)
     {
-        return try_fn([&]() -> KFR_DFT_REAL_PLAN_F32* { not_provided("multidimensional real DFT"); },
-                      (KFR_DFT_REAL_PLAN_F32*)nullptr);
+        return try_fn(
+            [&] {
+                return reinterpret_cast<KFR_DFT_REAL_PLAN_F32*>(
+                    new real_obj<float>(std::vector<size_t>{ size1, size2 }, (bool)real_out_is_enough));
+            },
+            (KFR_DFT_REAL_PLAN_F32*)nullptr);
     }
-    KFR_DFT_REAL_PLAN_F64* kfr_dft_real_create_2d_plan_f64(size_t, size_t, kfr_bool)
+    KFR_DFT_REAL_PLAN_F64* kfr_dft_real_create_2d_plan_f64(size_t size1, size_t size2, kfr_bool real_out_is_enough)
     {
-        return try_fn([&]() -> KFR_DFT_REAL_PLAN_F64* { not_provided("multidimensional real DFT"); },
-                      (KFR_DFT_REAL_PLAN_F64*)nullptr);
+        return try_fn(
+            [&] {
+                return reinterpret_cast<KFR_DFT_REAL_PLAN_F64*>(
+                    new real_obj<double>(std::vector<size_t>{ size1, size2 }, (bool)real_out_is_enough));
+            },
+            (KFR_DFT_REAL_PLAN_F64*)nullptr);
     }
-    KFR_DFT_REAL_PLAN_F32* kfr_dft_real_create_3d_plan_f32(size_t, size_t, size_t, kfr_bool)
+    KFR_DFT_REAL_PLAN_F32* kfr_dft_real_create_3d_plan_f32(size_t size1, size_t size2, size_t size3, kfr_bool real_out_is_enough)
     {
-        return try_fn([&]() -> KFR_DFT_REAL_PLAN_F32* { not_provided("multidimensional real DFT"); },
-                      (KFR_DFT_REAL_PLAN_F32*)nullptr);
+        return try_fn(
+            [&] {
+                return reinterpret_cast<KFR_DFT_REAL_PLAN_F32*>(
+                    new real_obj<float>(std::vector<size_t>{ size1, size2, size3 }, (bool)real_out_is_enough));
+            },
+            (KFR_DFT_REAL_PLAN_F32*)nullptr);
     }
-    KFR_DFT_REAL_PLAN_F64* kfr_dft_real_create_3d_plan_f64(size_t, size_t, size_t, kfr_bool)
+    KFR_DFT_REAL_PLAN_F64* kfr_dft_real_create_3d_plan_f64(size_t size1, size_t size2, size_t size3, kfr_bool real_out_is_enough)
     {
-        return try_fn([&]() -> KFR_DFT_REAL_PLAN_F64* { not_provided("multidimensional real DFT"); },
-                      (KFR_DFT_REAL_PLAN_F64*)nullptr);
+        return try_fn(
+            [&] {
+                return reinterpret_cast<KFR_DFT_REAL_PLAN_F64*>(
+                    new real_obj<double>(std::vector<size_t>{ size1, size2, size3 }, (bool)real_out_is_enough));
+            },
+            (KFR_DFT_REAL_PLAN_F64*)nullptr);
     }
-    KFR_DFT_REAL_PLAN_F32* kfr_dft_real_create_md_plan_f32(size_t, const unsigned*, kfr_bool)
+    KFR_DFT_REAL_PLAN_F32* kfr_dft_real_create_md_plan_f32(size_t dims, const unsigned* shape, kfr_bool real_out_is_enough)
     {
-        return try_fn([&]() -> KFR_DFT_REAL_PLAN_F32* { not_provided("multidimensional real DFT"); },
-                      (KFR_DFT_REAL_PLAN_F32*)nullptr);
+        return try_fn(
+            [&] { return reinterpret_cast<KFR_DFT_REAL_PLAN_F32*>(new real_obj<float>(shape_of(dims, shape), (bool)real_out_is_enough)); },
+            (KFR_DFT_REAL_PLAN_F32*)nullptr);
     }
-    KFR_DFT_REAL_PLAN_F64* kfr_dft_real_create_md_plan_f64(size_t, const unsigned*, kfr_bool)
+    KFR_DFT_REAL_PLAN_F64* kfr_dft_real_create_md_plan_f64(size_t dims, const unsigned* shape, kfr_bool real_out_is_enough)
     {
-        return try_fn([&]() -> KFR_DFT_REAL_PLAN_F64* { not_provided("multidimensional real DFT"); },
-                      (KFR_DFT_REAL_PLAN_F64*)nullptr);
+        return try_fn(
+            [&] { return reinterpret_cast<KFR_DFT_REAL_PLAN_F64*>(new real_obj<double>(shape_of(dims, shape), (bool)real_out_is_enough)); },
+            (KFR_DFT_REAL_PLAN_F64*)nullptr);
     }
     void kfr_dft_real_dump_f32(KFR_DFT_REAL_PLAN_F32* plan)
     {
-        try_void([&] { print_dump(real_cast<float>(plan)->desc, real_cast<float>(plan)->g.schedule()); });
+        try_void([&] { print_dump(real_cast<float>(plan)->desc, real_cast<float>(plan)->schedule()); });
     }
     void kfr_dft_real_dump_f64(KFR_DFT_REAL_PLAN_F64* plan)
     {
-        try_void([&] { print_dump(real_cast<double>(plan)->desc, real_cast<double>(plan)->g.schedule()); });
+        try_void([&] { print_dump(real_cast<double>(plan)->desc, real_cast<double>(plan)->schedule()); });
     }
     size_t kfr_dft_real_get_size_f32(KFR_DFT_REAL_PLAN_F32* plan)
     {
-        return try_fn([&] { return real_cast<float>(plan)->g.size(); }, (size_t)0);
+        return try_fn([&] { return real_cast<float>(plan)->size(); }, (size_t)0);
     }
     size_t kfr_dft_real_get_size_f64(KFR_DFT_REAL_PLAN_F64* plan)
     {
-        return try_fn([&] { return real_cast<double>(plan)->g.size(); }, (size_t)0);
+        return try_fn([&] { return real_cast<double>(plan)->size(); }, (size_t)0);
     }
     size_t kfr_dft_real_get_temp_size_f32(KFR_DFT_REAL_PLAN_F32* plan)
     {
-        return try_fn([&] { return real_cast<float>(plan)->desc.temp_size; }, (size_t)0);
+        return try_fn([&] { return real_cast<float>(plan)->temp_size(); }, (size_t)0);
     }
     size_t kfr_dft_real_get_temp_size_f64(KFR_DFT_REAL_PLAN_F64* plan)
     {
-        return try_fn([&] { return real_cast<double>(plan)->desc.temp_size; }, (size_t)0);
+        return try_fn([&] { return real_cast<double>(plan)->temp_size(); }, (size_t)0);
     }
     void kfr_dft_real_execute_f32(KFR_DFT_REAL_PLAN_F32* plan, kfr_c32* out, const kfr_f32* in, uint8_t*)
     {
@@ -525,11 +650,11 @@ extern "C"
     }
     void kfr_dct_dump_f32(KFR_DCT_PLAN_F32* plan)
     {
-        try_void([&] { print_dump(dct_cast<float>(plan)->desc, dct_cast<float>(plan)->g.schedule()); });
+        try_void([&] { print_dump({ dct_cast<float>(plan)->desc }, dct_cast<float>(plan)->g.schedule().text()); });
     }
     void kfr_dct_dump_f64(KFR_DCT_PLAN_F64* plan)
     {
-        try_void([&] { print_dump(dct_cast<double>(plan)->desc, dct_cast<double>(plan)->g.schedule()); });
+        try_void([&] { print_dump({ dct_cast<double>(plan)->desc }, dct_cast<double>(plan)->g.schedule().text()); });
     }
     size_t kfr_dct_get_size_f32(KFR_DCT_PLAN_F32* plan)
     {
@@ -649,7 +774,7 @@ extern "C"
         return try_fn(
             [&]
             {
-                dft_cast<float>(plan)->g.execute(out, in, (long long)batch, inverse != 0, as_stream(stream));
+                dft_cast<float>(plan)->execute(out, in, (long long)batch, inverse != 0, as_stream(stream));
                 return 0;
             },
             -1);
@@ -660,7 +785,7 @@ extern "C"
         return try_fn(
             [&]
             {
-                dft_cast<double>(plan)->g.execute(out, in, (long long)batch, inverse != 0, as_stream(stream));
+                dft_cast<double>(plan)->execute(out, in, (long long)batch, inverse != 0, as_stream(stream));
                 return 0;
             },
             -1);
@@ -673,9 +798,9 @@ extern "C"
             {
                 auto* o = real_cast<float>(plan);
                 if (inverse)
-                    o->g.inverse(out, in, (long long)batch, as_stream(stream));
+                    o->inverse(out, in, (long long)batch, as_stream(stream));
                 else
-                    o->g.forward(out, in, (long long)batch, as_stream(stream));
+                    o->forward(out, in, (long long)batch, as_stream(stream));
                 return 0;
             },
             -1);
@@ -688,9 +813,9 @@ extern "C"
             {
                 auto* o = real_cast<double>(plan);
                 if (inverse)
-                    o->g.inverse(out, in, (long long)batch, as_stream(stream));
+                    o->inverse(out, in, (long long)batch, as_stream(stream));
                 else
-                    o->g.forward(out, in, (long long)batch, as_stream(stream));
+                    o->forward(out, in, (long long)batch, as_stream(stream));
                 return 0;
             },
             -1);
@@ -719,11 +844,11 @@ extern "C"
     }
     size_t kfr_amd_dft_real_get_complex_size_f32(KFR_DFT_REAL_PLAN_F32* plan)
     {
-        return try_fn([&] { return real_cast<float>(plan)->g.complex_size(); }, (size_t)0);
+        return try_fn([&] { return real_cast<float>(plan)->complex_size(); }, (size_t)0);
     }
     size_t kfr_amd_dft_real_get_complex_size_f64(KFR_DFT_REAL_PLAN_F64* plan)
     {
-        return try_fn([&] { return real_cast<double>(plan)->g.complex_size(); }, (size_t)0);
+        return try_fn([&] { return real_cast<double>(plan)->complex_size(); }, (size_t)0);
     }
 
     long kfr_amd_convolve(int dtype, int mode, const void* a, size_t na, const void* b, size_t nb, void* out)

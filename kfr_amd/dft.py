@@ -162,6 +162,75 @@ class dft_plan_real:
             self._h = None
 
 
+class dft_plan_md:
+    """Complex DFT over every axis of a row-major array (include/kfr/dft/fft.hpp:569-691), through
+    kfr_dft_create_md_plan_*; executed with the same kfr_dft_execute_* calls as one-axis plans."""
+
+    def __init__(self, shape, dtype=np.complex64):
+        self.t = _tag(dtype)
+        self.shape = tuple(int(v) for v in shape)
+        sh = (C.c_uint * len(self.shape))(*self.shape)
+        L = lib()
+        self._h = getattr(L, f"kfr_dft_create_md_plan_{self.t}")(len(self.shape), C.cast(sh, C.c_void_p))
+        if not self._h:
+            raise KfrError(L.kfr_last_error().decode())
+        self.size = getattr(L, f"kfr_dft_get_size_{self.t}")(self._h)
+        self.temp_size = getattr(L, f"kfr_dft_get_temp_size_{self.t}")(self._h)
+
+    def execute(self, out, in_, temp=None, inverse: bool = False):
+        fn = f"kfr_dft_execute_inverse_{self.t}" if inverse else f"kfr_dft_execute_{self.t}"
+        getattr(lib(), fn)(self._h, _ptr(out), _ptr(in_), None if temp is None else _ptr(temp))
+        check()
+        return out
+
+    def dump(self):
+        getattr(lib(), f"kfr_dft_dump_{self.t}")(self._h)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            getattr(lib(), f"kfr_dft_delete_plan_{self.t}")(h)
+            self._h = None
+
+
+class dft_plan_md_real:
+    """Real DFT over every axis of a row-major array (include/kfr/dft/fft.hpp:693-924): real input of
+    `shape`, complex output of `complex_size()` (the last axis n/2+1, CCs), and back."""
+
+    def __init__(self, shape, real_out_is_enough: bool = False, dtype=np.float32):
+        self.t = _tag(dtype)
+        self.shape = tuple(int(v) for v in shape)
+        self.real_out_is_enough = bool(real_out_is_enough)
+        sh = (C.c_uint * len(self.shape))(*self.shape)
+        L = lib()
+        self._h = getattr(L, f"kfr_dft_real_create_md_plan_{self.t}")(len(self.shape), C.cast(sh, C.c_void_p),
+                                                                      self.real_out_is_enough)
+        if not self._h:
+            raise KfrError(L.kfr_last_error().decode())
+        self.size = getattr(L, f"kfr_dft_real_get_size_{self.t}")(self._h)
+        self.temp_size = getattr(L, f"kfr_dft_real_get_temp_size_{self.t}")(self._h)
+
+    def complex_size(self) -> tuple:
+        return self.shape[:-1] + (self.shape[-1] // 2 + 1,)
+
+    def execute(self, out, in_, temp=None):
+        """Forward when `in_` is real, inverse (real output) when it is complex."""
+        is_complex = np.dtype(in_.dtype).kind == "c" if isinstance(in_, np.ndarray) else in_.is_complex()
+        fn = f"kfr_dft_real_execute_inverse_{self.t}" if is_complex else f"kfr_dft_real_execute_{self.t}"
+        getattr(lib(), fn)(self._h, _ptr(out), _ptr(in_), None if temp is None else _ptr(temp))
+        check()
+        return out
+
+    def dump(self):
+        getattr(lib(), f"kfr_dft_real_dump_{self.t}")(self._h)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            getattr(lib(), f"kfr_dft_real_delete_plan_{self.t}")(h)
+            self._h = None
+
+
 class dct_plan:
     """DCT-II (inverse: DCT-III with x[0] halved), unscaled (include/kfr/dft/fft.hpp:927-978)."""
 

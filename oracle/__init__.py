@@ -145,6 +145,10 @@ class _Ref:
         L.kref_filter_apply.argtypes = [vp, vp, vp, sz]
         L.kref_filter_reset.argtypes = [vp]
         L.kref_filter_delete.argtypes = [vp]
+        L.kref_dft_md.argtypes = [C.c_int, sz, vp, vp, vp, C.c_int]
+        L.kref_dft_md.restype = C.c_long
+        L.kref_rdft_md.argtypes = [C.c_int, sz, vp, C.c_int, vp, vp, C.c_int]
+        L.kref_rdft_md.restype = C.c_long
         L.kref_time.argtypes = [C.c_int, C.c_int, sz, sz, C.c_int, C.c_int, C.c_int, vp, vp]
         L.kref_time.restype = C.c_double
         self.L = L
@@ -198,6 +202,35 @@ class _Ref:
         self.L.kref_dct(0 if x.dtype == np.float32 else 1, n, x.size // n, out.ctypes.data, x.ctypes.data,
                         int(inverse), threads)
         return out
+
+    def dft_md(self, x, inverse=False):
+        """(output, temp_size) of the reference dft_plan_md over every axis of x."""
+        x = np.ascontiguousarray(x)
+        out = np.empty_like(x)
+        sh = np.ascontiguousarray(x.shape, dtype=np.uint32)
+        t = self.L.kref_dft_md(0 if x.dtype == np.complex64 else 1, x.ndim, sh.ctypes.data, out.ctypes.data,
+                               x.ctypes.data, int(inverse))
+        return out, t
+
+    def rdft_md(self, x, real_out_is_enough=False):
+        """(complex output, temp_size) of the reference dft_plan_md_real, forward."""
+        x = np.ascontiguousarray(x)
+        sh = np.ascontiguousarray(x.shape, dtype=np.uint32)
+        cdt = np.complex64 if x.dtype == np.float32 else np.complex128
+        out = np.empty(x.shape[:-1] + (x.shape[-1] // 2 + 1,), dtype=cdt)
+        t = self.L.kref_rdft_md(0 if x.dtype == np.float32 else 1, x.ndim, sh.ctypes.data, int(real_out_is_enough),
+                                out.ctypes.data, x.ctypes.data, 0)
+        return out, t
+
+    def irdft_md(self, X, shape, real_out_is_enough=False):
+        """(real output, temp_size) of the reference dft_plan_md_real, inverse (unscaled)."""
+        X = np.ascontiguousarray(X)
+        sh = np.ascontiguousarray(shape, dtype=np.uint32)
+        rdt = np.float32 if X.dtype == np.complex64 else np.float64
+        out = np.empty(X.size * 2, dtype=rdt)  # room for real_out_size()
+        t = self.L.kref_rdft_md(0 if rdt == np.float32 else 1, len(shape), sh.ctypes.data, int(real_out_is_enough),
+                                out.ctypes.data, X.ctypes.data, 1)
+        return out[: int(np.prod(shape))].reshape(shape), t
 
     def convolve(self, a, b=None, mode=0):
         a = np.ascontiguousarray(a)

@@ -6,6 +6,7 @@
 //   dft_plan<T>                 include/kfr/dft/fft.hpp:183-456
 //   dft_plan_real<T>            include/kfr/dft/fft.hpp:459-566
 //   dct_plan<T>                 include/kfr/dft/fft.hpp:927-978
+//   dft_plan_md / _md_real      include/kfr/dft/fft.hpp:569-924
 //   convolve/correlate          include/kfr/dft/convolution.hpp:53-76
 //   convolve_filter<T>          include/kfr/dft/convolution.hpp:99-148
 // The plan description walks dft_plan::all_stages, the same fields dft_stage::dump() prints
@@ -258,6 +259,61 @@ KREF_API int kref_dft(int dtype, size_t n, size_t batch, void* out, const void* 
         else
             run_dft<double>(n, batch, out, in, inverse, threads);
         return 0;
+    }
+    catch (...)
+    {
+        return -1;
+    }
+}
+
+// dft_plan_md / dft_plan_md_real (include/kfr/dft/fft.hpp:569-924) on one array; returns the plan's
+// temp_size, or -1. The real inverse with real_out_is_enough needs `out` to hold real_out_size() values.
+template <typename T>
+long run_dft_md(size_t dims, const unsigned* shp, void* out, const void* in, int inverse)
+{
+    shape<dynamic_shape> sh(dims);
+    for (size_t i = 0; i < dims; ++i)
+        sh[i] = shp[i];
+    dft_plan_md<T, dynamic_shape> plan(sh);
+    std::vector<u8> temp(plan.temp_size + 64);
+    plan.execute(static_cast<complex<T>*>(out), static_cast<const complex<T>*>(in), temp.data(), inverse != 0);
+    return (long)plan.temp_size;
+}
+template <typename T>
+long run_rdft_md(size_t dims, const unsigned* shp, int real_out_is_enough, void* out, const void* in, int inverse)
+{
+    shape<dynamic_shape> sh(dims);
+    for (size_t i = 0; i < dims; ++i)
+        sh[i] = shp[i];
+    dft_plan_md_real<T, dynamic_shape> plan(sh, real_out_is_enough != 0);
+    std::vector<u8> temp(plan.temp_size + 64);
+    if (inverse)
+        plan.execute(static_cast<T*>(out), static_cast<const complex<T>*>(in), temp.data(), cinvert_t{});
+    else
+        plan.execute(static_cast<complex<T>*>(out), static_cast<const T*>(in), temp.data(), cdirect_t{});
+    return (long)plan.temp_size;
+}
+
+KREF_API long kref_dft_md(int dtype, size_t dims, const unsigned* shape, void* out, const void* in, int inverse)
+{
+    try
+    {
+        return dtype == 0 ? run_dft_md<float>(dims, shape, out, in, inverse)
+                          : run_dft_md<double>(dims, shape, out, in, inverse);
+    }
+    catch (...)
+    {
+        return -1;
+    }
+}
+
+KREF_API long kref_rdft_md(int dtype, size_t dims, const unsigned* shape, int real_out_is_enough, void* out,
+                           const void* in, int inverse)
+{
+    try
+    {
+        return dtype == 0 ? run_rdft_md<float>(dims, shape, real_out_is_enough, out, in, inverse)
+                          : run_rdft_md<double>(dims, shape, real_out_is_enough, out, in, inverse);
     }
     catch (...)
     {
