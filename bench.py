@@ -152,6 +152,59 @@ def uneven_batch(torch, kfr_amd, stream, reps: int, check: bool):
     return res
 
 
+def precision_config(torch, kfr_amd, stream, reps: int):
+    """Complex C2C N = 2^20, batch 64, unit-variance complex Gaussian data (seed 4) stored in bfloat16 and
+    binary16 (radix-16 passes on MFMA) against the FP32 path on the same values; errors against the
+    FP64 transform of the stored input, on 4 sampled transforms."""
+    n, batch = 1 << 20, 64
+    g = torch.Generator(device="cuda")
+    g.manual_seed(4)
+    x32 = torch.randn(batch, n, 2, device="cuda", generator=g) * (0.5 ** 0.5)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    def errors(x, y):
+        xc = torch.view_as_complex(x[:4].double().contiguous()).cpu().numpy()
+        yc = torch.view_as_complex(y[:4].double().contiguous()).cpu().numpy()
+        ref = np.fft.fft(xc, axis=-1)
+        rel = np.sqrt(np.sum(np.abs(yc - ref) ** 2, axis=-1) / np.sum(np.abs(ref) ** 2, axis=-1))
+        return float(f"{rel.max():.3e}"), float(f"{np.max(np.abs(yc - ref)) / np.max(np.abs(ref)):.3e}")
+
+    res = {"workload": "complex C2C N=2^20 x 64, unit-variance complex Gaussian (seed 4), stored bf16 / f16 vs f32"}
+    p32 = kfr_amd.dft_plan(n, np.complex64)
+    out = {}
+    for name, dt in (("f32", torch.float32), ("bf16", torch.bfloat16), ("f16", torch.float16)):
+        x = x32.to(dt)
+        y = torch.empty_like(x)
+        if name == "f32":
+            fn = lambda x=x, y=y: p32.execute_batch(y, x, batch, False, stream)  # noqa: E731
+        else:
+            ph = kfr_amd.dft_plan_half(n, name)
+            fn = lambda x=x, y=y, ph=ph: ph.execute_batch(y, x, batch, False, stream)  # noqa: E731
+        t = timed(fn)
+        rel, mx = errors(x, y)
+        bytes_alg = 2 * (8 if name == "f32" else 4) * n * batch
+        out[name] = {"ms": round(t, 4), "gbs": round(bytes_alg / (t * 1e-3) * 1e-9, 1),
+                     "frac": round(bytes_alg / (t * 1e-3) * 1e-9 / HBM_PEAK_GBS, 4),
+                     "gflops": round(5.0 * n * 20 * batch / (t * 1e-3) * 1e-9, 1),
+                     "rel_rms_vs_fp64_max": rel, "max_err_rel": mx}
+        if name != "f32":
+            out[name]["schedule"] = ph.describe()
+    for name in ("bf16", "f16"):
+        out[name]["speedup_vs_f32"] = round(out["f32"]["ms"] / out[name]["ms"], 3)
+    res.update(out)
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,6 +215,7 @@ def main() -> int:
     ap.add_argument("--only", default=None,
                     help="comma-separated subset of configurations kind:dtype:n (e.g. c2c:f32:4096), for profiling")
     ap.add_argument("--no-uneven", action="store_true", help="skip the mixed-length batch measurement")
+    ap.add_argument("--no-precision", action="store_true", help="skip the half-precision measurement")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one plain 256 MiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -309,6 +363,10 @@ def main() -> int:
     if not args.no_uneven and not args.only:
         uneven = uneven_batch(torch, kfr_amd, stream, max(3, args.steps), check=True)
 
+    precision = None
+    if not args.no_precision and not args.only:
+        precision = precision_config(torch, kfr_amd, stream, max(3, args.steps))
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(cfgs)
@@ -333,6 +391,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "rel_rms_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs.items()},
         "uneven_batch": uneven,
+        "precision": precision,
         "fourstep_c2c_f32_frac": round(fs_bytes / (fs_t * 1e-3) * 1e-9 / HBM_PEAK_GBS, 4) if fs_t else None,
         "wall_s": round(wall, 3),
         "per_config": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in p.items()} for p in per],

@@ -244,6 +244,20 @@ KFR_AMD_API int kfr_amd_batch_execute(KFR_AMD_BATCH* batch, void* out, const voi
 KFR_AMD_API int kfr_amd_batch_describe(KFR_AMD_BATCH* batch, char* buf, size_t buflen);
 KFR_AMD_API void kfr_amd_batch_delete(KFR_AMD_BATCH* batch);
 
+/* Complex transforms of power-of-two size stored in half precision: each value is a pair (re, im) of
+ * bfloat16 (storage 1) or IEEE binary16 (storage 2), 4 bytes per complex value; arithmetic is FP32 and
+ * the radix-16 passes run as 16-point DFT-matrix products on MFMA with the matrix and the data split
+ * into bf16 hi + lo parts (FP32 accumulation). Transforms longer than one workgroup keep their
+ * intermediate in binary16 with a power-of-two scale per group of columns. Unscaled, in place allowed,
+ * contiguous batches; device pointers run on `stream` without synchronising, host pointers are staged
+ * synchronously. No counterpart in the reference (whose transforms are float and double only). */
+typedef struct KFR_AMD_DFT_HALF { int opaque; } KFR_AMD_DFT_HALF;
+KFR_AMD_API KFR_AMD_DFT_HALF* kfr_amd_dft_half_create(int storage, size_t size);
+KFR_AMD_API int kfr_amd_dft_half_execute_batch(KFR_AMD_DFT_HALF* plan, void* out, const void* in, size_t batch,
+                                               int inverse, void* stream);
+KFR_AMD_API int kfr_amd_dft_half_describe(KFR_AMD_DFT_HALF* plan, char* buf, size_t buflen);
+KFR_AMD_API void kfr_amd_dft_half_delete(KFR_AMD_DFT_HALF* plan);
+
 /* The reference plan descriptor for a size, computed on the host without a GPU: text in the layout of
  * dft_plan::dump() (one line per stage) preceded by a summary line. kind: 0 dft_plan, 1 dft_plan_real
  * (arg = pack format), 2 dct_plan, 3 dft_plan with progressive_optimized. Returns the text length

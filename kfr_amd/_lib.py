@@ -91,6 +91,10 @@ def _proto(lib):
     f("kfr_amd_batch_execute", C.c_int, vp, vp, vp, vp)
     f("kfr_amd_batch_describe", C.c_int, vp, C.c_char_p, c_size)
     f("kfr_amd_batch_delete", None, vp)
+    f("kfr_amd_dft_half_create", vp, C.c_int, c_size)
+    f("kfr_amd_dft_half_execute_batch", C.c_int, vp, vp, vp, c_size, C.c_int, vp)
+    f("kfr_amd_dft_half_describe", C.c_int, vp, C.c_char_p, c_size)
+    f("kfr_amd_dft_half_delete", None, vp)
     f("kfr_amd_inject_alloc_failure", None, C.c_int)
     f("kfr_amd_synchronize", C.c_int)
 

@@ -4,6 +4,7 @@
 #include "fft_batch.hpp"
 #include "fft_engine.hpp"
 #include "fft_md.hpp"
+#include "fft_pow2.hpp"
 #include "plan_desc.hpp"
 
 #include <hip/hip_runtime.h>
@@ -387,6 +388,14 @@ struct batch_obj
         else
             d->execute(out, in, s);
     }
+};
+
+struct half_obj
+{
+    std::unique_ptr<pow2_fft<float>> f;
+    size_t n = 0;
+    dev_mem buf; // staging for host pointers
+    std::mutex mu;
 };
 
 int copy_text(const std::string& s, char* buf, size_t buflen)
@@ -1095,6 +1104,71 @@ extern "C"
     void kfr_amd_batch_delete(KFR_AMD_BATCH* batch)
     {
         try_void([&] { delete reinterpret_cast<batch_obj*>(batch); });
+    }
+
+    KFR_AMD_DFT_HALF* kfr_amd_dft_half_create(int storage, size_t size)
+    {
+        return try_fn(
+            [&]
+            {
+                if (storage != 1 && storage != 2)
+                    throw std::invalid_argument("storage must be 1 (bf16) or 2 (f16)");
+                if (size < 2 || (size & (size - 1)) || !pow2_fft<float>::supports(size))
+                    throw std::invalid_argument("half-precision transforms take a power-of-two size from 2 to 2^28");
+                auto* h = new half_obj();
+                try
+                {
+                    h->n = size;
+                    h->f.reset(new pow2_fft<float>(size, library_stream(), storage));
+                    hip_check(hipStreamSynchronize(library_stream()), "plan initialisation");
+                }
+                catch (...)
+                {
+                    delete h;
+                    throw;
+                }
+                return reinterpret_cast<KFR_AMD_DFT_HALF*>(h);
+            },
+            (KFR_AMD_DFT_HALF*)nullptr);
+    }
+    int kfr_amd_dft_half_execute_batch(KFR_AMD_DFT_HALF* plan, void* out, const void* in, size_t batch, int inverse,
+                                       void* stream)
+    {
+        return try_fn(
+            [&]
+            {
+                check_plan(plan);
+                auto* h = reinterpret_cast<half_obj*>(plan);
+                std::lock_guard<std::mutex> lock(h->mu);
+                if (is_device_ptr(in) && is_device_ptr(out))
+                {
+                    h->f->run(inverse != 0, in, out, (long long)batch, as_stream(stream));
+                    return 0;
+                }
+                const size_t bytes = batch * h->n * 4;
+                hipStream_t s      = library_stream();
+                h->buf.ensure(bytes);
+                hip_check(hipMemcpyAsync(h->buf.ptr, in, bytes, hipMemcpyDefault, s), "input copy");
+                h->f->run(inverse != 0, h->buf.ptr, h->buf.ptr, (long long)batch, s);
+                hip_check(hipMemcpyAsync(out, h->buf.ptr, bytes, hipMemcpyDefault, s), "output copy");
+                hip_check(hipStreamSynchronize(s), "execute");
+                return 0;
+            },
+            -1);
+    }
+    int kfr_amd_dft_half_describe(KFR_AMD_DFT_HALF* plan, char* buf, size_t buflen)
+    {
+        return try_fn(
+            [&]
+            {
+                check_plan(plan);
+                return copy_text("gpu(gfx950): " + reinterpret_cast<half_obj*>(plan)->f->describe(), buf, buflen);
+            },
+            -1);
+    }
+    void kfr_amd_dft_half_delete(KFR_AMD_DFT_HALF* plan)
+    {
+        try_void([&] { delete reinterpret_cast<half_obj*>(plan); });
     }
 
     void kfr_amd_inject_alloc_failure(int n) { set_alloc_failure_countdown(n); }

@@ -137,16 +137,195 @@ __device__ __forceinline__ void load_table(const cx<T>* p, cx<T> (&w)[R])
 template <typename T>
 struct args
 {
-    const cx<T>* in  = nullptr;
-    cx<T>* out       = nullptr;
+    const void* in   = nullptr;
+    void* out        = nullptr;
     const cx<T>* tw  = nullptr; // pass tables of this kernel's length
     const cx<T>* ta  = nullptr; // four-step: W_n^(h*2^lbits)
     const cx<T>* tb  = nullptr; // four-step: W_n^l, l < 2^lbits
+    int* scales      = nullptr; // half-precision four-step: power-of-two scale of each column group
     long long count  = 0;       // transforms
     int lo           = 0;       // four-step: log2 of the other dimension (col: n1, row: n2)
     int lbits        = 0;
+    int gshift       = 0; // half-precision row kernel: log2 of the column kernel's columns per workgroup
     unsigned nblk    = 0; // workgroups with work (the grid of split kernels is a multiple of 8)
 };
+
+// ---- storage precision of the data in HBM (the arithmetic is T throughout)
+//   prec_full: cx<T>; prec_bf16 / prec_f16: complex values of two bfloat16 / binary16 (T = float only).
+// The half-precision four-step keeps its intermediate as binary16 with one power-of-two scale per
+// column-kernel workgroup, so values of any range keep 11-bit precision.
+enum storage_prec : int
+{
+    prec_full = 0,
+    prec_bf16 = 1,
+    prec_f16  = 2
+};
+
+template <typename H>
+struct alignas(2 * sizeof(H)) hcx
+{
+    H re, im;
+};
+
+template <typename T, int PREC>
+__device__ __forceinline__ cx<T> ld_elem(const void* p, long long i)
+{
+    if constexpr (PREC == prec_full)
+        return static_cast<const cx<T>*>(p)[i];
+    else if constexpr (PREC == prec_bf16)
+    {
+        const hcx<__bf16> h = static_cast<const hcx<__bf16>*>(p)[i];
+        return { (T)(float)h.re, (T)(float)h.im };
+    }
+    else
+    {
+        const hcx<_Float16> h = static_cast<const hcx<_Float16>*>(p)[i];
+        return { (T)(float)h.re, (T)(float)h.im };
+    }
+}
+
+template <typename T, int PREC>
+__device__ __forceinline__ void st_elem(void* p, long long i, cx<T> v)
+{
+    if constexpr (PREC == prec_full)
+        static_cast<cx<T>*>(p)[i] = v;
+    else if constexpr (PREC == prec_bf16)
+        static_cast<hcx<__bf16>*>(p)[i] = { (__bf16)(float)v.re, (__bf16)(float)v.im };
+    else
+        static_cast<hcx<_Float16>*>(p)[i] = { (_Float16)(float)v.re, (_Float16)(float)v.im };
+}
+
+// precision of the four-step intermediate
+template <int PREC>
+inline constexpr int mid_prec = PREC == prec_full ? prec_full : prec_f16;
+
+// ---- radix-16 butterflies on MFMA (half-precision storage): the 16-point DFT of 32 butterflies is
+// the real product [Re y; Im y] = [[Re F, -Im F], [Im F, Re F]] [Re x; Im x] with the 32x32 matrix as
+// the A operand of v_mfma_f32_32x32x16_bf16 (lane l, slot s: A[l%32][8(l/32)+s]; B[8(l/32)+s][l%32];
+// D slot v: D[8(v/4)+4(l/32)+v%4][l%32]). Matrix and data are each split into bf16 hi + lo parts and
+// three products are accumulated in FP32 (hi*hi + hi*lo + lo*hi), about 16 significant bits.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+struct mfma16
+{
+    bf16x8 a0h, a0l, a1h, a1l; // chunk 0 multiplies Re x, chunk 1 Im x
+};
+
+__device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo)
+{
+    hi = (__bf16)x;
+    lo = (__bf16)(x - (float)hi);
+}
+
+template <bool INV>
+__device__ __forceinline__ mfma16 make_mfma16()
+{
+    // cos and sin of 2*pi*q/16
+    constexpr float cq[16] = { (float)ccos2pi(0, 16),  (float)ccos2pi(1, 16),  (float)ccos2pi(2, 16),
+                               (float)ccos2pi(3, 16),  (float)ccos2pi(4, 16),  (float)ccos2pi(5, 16),
+                               (float)ccos2pi(6, 16),  (float)ccos2pi(7, 16),  (float)ccos2pi(8, 16),
+                               (float)ccos2pi(9, 16),  (float)ccos2pi(10, 16), (float)ccos2pi(11, 16),
+                               (float)ccos2pi(12, 16), (float)ccos2pi(13, 16), (float)ccos2pi(14, 16),
+                               (float)ccos2pi(15, 16) };
+    constexpr float sq[16] = { (float)csin2pi(0, 16),  (float)csin2pi(1, 16),  (float)csin2pi(2, 16),
+                               (float)csin2pi(3, 16),  (float)csin2pi(4, 16),  (float)csin2pi(5, 16),
+                               (float)csin2pi(6, 16),  (float)csin2pi(7, 16),  (float)csin2pi(8, 16),
+                               (float)csin2pi(9, 16),  (float)csin2pi(10, 16), (float)csin2pi(11, 16),
+                               (float)csin2pi(12, 16), (float)csin2pi(13, 16), (float)csin2pi(14, 16),
+                               (float)csin2pi(15, 16) };
+    const int l      = __lane_id();
+    const int i      = l & 31;
+    const int h      = l >> 5;
+    const int m      = i & 15;
+    const bool imrow = i >= 16;
+    mfma16 K;
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+    {
+        const int q    = (m * (8 * h + s)) & 15;
+        const float c  = cq[q];
+        const float sg = INV ? -sq[q] : sq[q]; // forward F = cos - i sin
+        // row i < 16: Re y_m = sum c Re x + sg Im x ; row i >= 16: Im y_m = sum -sg Re x + c Im x
+        const float a0 = imrow ? -sg : c;
+        const float a1 = imrow ? c : sg;
+        __bf16 hi, lo;
+        split_bf16(a0, hi, lo);
+        K.a0h[s] = hi;
+        K.a0l[s] = lo;
+        split_bf16(a1, hi, lo);
+        K.a1h[s] = hi;
+        K.a1l[s] = lo;
+    }
+    return K;
+}
+
+// 16-point DFTs of the 32 butterflies whose inputs r in [8h, 8h+8) this lane holds in g
+__device__ __forceinline__ f32x16 mfma16_group(const cx<float> (&g)[8], const mfma16& K)
+{
+    bf16x8 b0h, b0l, b1h, b1l;
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+    {
+        __bf16 hi, lo;
+        split_bf16(g[s].re, hi, lo);
+        b0h[s] = hi;
+        b0l[s] = lo;
+        split_bf16(g[s].im, hi, lo);
+        b1h[s] = hi;
+        b1l[s] = lo;
+    }
+    f32x16 acc = {};
+    acc        = __builtin_amdgcn_mfma_f32_32x32x16_bf16(K.a0h, b0h, acc, 0, 0, 0);
+    acc        = __builtin_amdgcn_mfma_f32_32x32x16_bf16(K.a0h, b0l, acc, 0, 0, 0);
+    acc        = __builtin_amdgcn_mfma_f32_32x32x16_bf16(K.a0l, b0h, acc, 0, 0, 0);
+    acc        = __builtin_amdgcn_mfma_f32_32x32x16_bf16(K.a1h, b1h, acc, 0, 0, 0);
+    acc        = __builtin_amdgcn_mfma_f32_32x32x16_bf16(K.a1h, b1l, acc, 0, 0, 0);
+    acc        = __builtin_amdgcn_mfma_f32_32x32x16_bf16(K.a1l, b1h, acc, 0, 0, 0);
+    return acc;
+}
+
+// value of lane l ^ 32: v_permlane32_swap exchanges the upper half-wave of its first operand with the
+// lower half-wave of its second, without going through LDS
+__device__ __forceinline__ float xchg32(float x)
+{
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+    return __uint_as_float(__lane_id() < 32 ? r[1] : r[0]);
+}
+
+// Wave-cooperative radix-16 butterfly: every lane of the wave holds the 16 inputs of its own
+// butterfly in u and gets its 16 outputs back. Lanes l and l+32 exchange halves so that each holds
+// inputs [8h, 8h+8) of butterflies l%32 (group 0) and 32 + l%32 (group 1), both groups run on MFMA,
+// and a second exchange returns every output to the lane that owns the butterfly.
+__device__ __forceinline__ void bfly16_mfma(cx<float> (&u)[16], const mfma16& K)
+{
+    const int h = __lane_id() >> 5;
+    cx<float> g0[8], g1[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+    {
+        const cx<float> snd = h ? u[s] : u[8 + s];
+        const cx<float> rcv = { xchg32(snd.re), xchg32(snd.im) };
+        g0[s]               = h ? rcv : u[s];
+        g1[s]               = h ? u[8 + s] : rcv;
+    }
+    const f32x16 d0 = mfma16_group(g0, K);
+    const f32x16 d1 = mfma16_group(g1, K);
+    // slot q holds output m = 4h + q (q < 4) or 8 + 4h + q - 4 (q >= 4): Re in d[q], Im in d[8 + q]
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+    {
+        const cx<float> o0  = { d0[q], d0[8 + q] };
+        const cx<float> o1  = { d1[q], d1[8 + q] };
+        const cx<float> snd = h ? o0 : o1;
+        const cx<float> rcv = { xchg32(snd.re), xchg32(snd.im) };
+        const cx<float> own = h ? o1 : o0;
+        const int m0        = q < 4 ? q : 4 + q;     // output index held by h = 0
+        const int m1        = q < 4 ? 4 + q : 8 + q; // output index held by h = 1
+        u[m0]               = h ? rcv : own;
+        u[m1]               = h ? own : rcv;
+    }
+}
 
 // Workgroup barrier for the LDS exchanges: the fences cover the local address space only, so it
 // does not wait for outstanding global memory operations.
@@ -158,8 +337,9 @@ __device__ __forceinline__ void lds_barrier()
 }
 
 // pass P of the schedule on the registers v (strided set in, strided set out after the exchange)
-template <typename T, int L, bool INV, int ROLE, int P>
-__device__ __forceinline__ void run_pass(regs<T, L>& v, cx<T>* lds, int tb, int t, const cx<T>* tw)
+template <typename T, int L, bool INV, int ROLE, int PREC, int P>
+__device__ __forceinline__ void run_pass(regs<T, L>& v, cx<T>* lds, int tb, int t, const cx<T>* tw,
+                                         const mfma16& K)
 {
     using S          = shape<T, L>;
     using G          = geom<T, L, ROLE>;
@@ -190,7 +370,10 @@ __device__ __forceinline__ void run_pass(regs<T, L>& v, cx<T>* lds, int tb, int 
 #pragma unroll
         for (int r = 0; r < R; ++r)
             u[r] = v[m + r * M];
-        butterfly<R, INV>(u);
+        if constexpr (PREC != prec_full && R == 16)
+            bfly16_mfma(u, K);
+        else
+            butterfly<R, INV>(u);
 #pragma unroll
         for (int r = 0; r < R; ++r)
             v[m + r * M] = u[r];
@@ -213,11 +396,20 @@ __device__ __forceinline__ void run_pass(regs<T, L>& v, cx<T>* lds, int tb, int 
         for (int k = 0; k < E; ++k)
             v[k] = lds[laddr<ROLE, G::W, G::LD>(tb, t + k * TP)];
         lds_barrier();
-        run_pass<T, L, INV, ROLE, P + 1>(v, lds, tb, t, tw);
+        run_pass<T, L, INV, ROLE, PREC, P + 1>(v, lds, tb, t, tw, K);
     }
 }
 
-template <typename T, int L, bool INV, int ROLE>
+// power-of-two scale 2^k of a value array whose largest |re|, |im| is mx: the scaled values stay below
+// 2^15, inside binary16's range
+__device__ __forceinline__ int mid_scale_exp(float mx)
+{
+    int e = 0;
+    (void)frexpf(mx, &e); // mx < 2^e
+    return mx > 0.f ? 15 - e : 0;
+}
+
+template <typename T, int L, bool INV, int ROLE, int PREC>
 __global__ void __launch_bounds__((geom<T, L, ROLE>::BS)) k_pow2(args<T> a)
 {
     using S          = shape<T, L>;
@@ -229,6 +421,9 @@ __global__ void __launch_bounds__((geom<T, L, ROLE>::BS)) k_pow2(args<T> a)
     constexpr int BS = G::BS;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+    mfma16 K{};
+    if constexpr (PREC != prec_full)
+        K = make_mfma16<INV>();
 
     unsigned blk = blockIdx.x;
     if constexpr (ROLE != role_direct)
@@ -254,12 +449,12 @@ __global__ void __launch_bounds__((geom<T, L, ROLE>::BS)) k_pow2(args<T> a)
         for (int k = 0; k < E; ++k)
         {
             const int e = tid + k * BS;
-            v[k]        = e < lim ? a.in[base + e] : cx<T>{ T(0), T(0) };
+            v[k]        = e < lim ? ld_elem<T, PREC>(a.in, base + e) : cx<T>{ T(0), T(0) };
         }
 #pragma unroll
         for (int k = 0; k < E; ++k)
         {
-            const int e    = tid + k * BS;
+            const int e       = tid + k * BS;
             lds[e + (e >> 4)] = v[k];
         }
         lds_barrier();
@@ -270,22 +465,42 @@ __global__ void __launch_bounds__((geom<T, L, ROLE>::BS)) k_pow2(args<T> a)
     }
     else if constexpr (ROLE == role_col)
     {
-        const long long bl  = g >> a.lo;
-        const long long col = g & ((1ll << a.lo) - 1);
-        const cx<T>* src    = a.in + (bl << (a.lo + L)) + col;
+        const long long bl   = g >> a.lo;
+        const long long col  = g & ((1ll << a.lo) - 1);
+        const long long base = (bl << (a.lo + L)) + col;
 #pragma unroll
         for (int k = 0; k < E; ++k)
-            v[k] = valid ? src[(long long)(t + k * TP) << a.lo] : cx<T>{ T(0), T(0) };
+            v[k] = valid ? ld_elem<T, PREC>(a.in, base + ((long long)(t + k * TP) << a.lo)) : cx<T>{ T(0), T(0) };
+    }
+    else if constexpr (ROLE == role_row && PREC != prec_full)
+    {
+        // binary16 intermediate: element (col = i) carries the scale of its column-kernel workgroup
+        const long long base = g * N;
+#pragma unroll
+        for (int k = 0; k < E; ++k)
+        {
+            v[k] = { T(0), T(0) };
+            if (valid)
+            {
+                const long long e  = base + t + k * TP;
+                const long long bl = g >> a.lo;
+                const long long gc = (bl << L) + t + k * TP; // column (bl, col) over the batch
+                const int sc      = a.scales[gc >> a.gshift];
+                const cx<T> h     = ld_elem<T, prec_f16>(a.in, e);
+                v[k]              = { ldexpf(h.re, -sc), ldexpf(h.im, -sc) };
+            }
+        }
     }
     else
     {
-        const cx<T>* src = a.in + g * N + t;
+        const long long base = g * N + t;
 #pragma unroll
         for (int k = 0; k < E; ++k)
-            v[k] = valid ? src[k * TP] : cx<T>{ T(0), T(0) };
+            v[k] = valid ? ld_elem<T, ROLE == role_row ? mid_prec<PREC> : PREC>(a.in, base + k * TP)
+                         : cx<T>{ T(0), T(0) };
     }
 
-    run_pass<T, L, INV, ROLE, 0>(v, lds, tb, t, a.tw);
+    run_pass<T, L, INV, ROLE, PREC, 0>(v, lds, tb, t, a.tw, K);
     // v[k] = X[t + k*TPT]
 
     if constexpr (ROLE == role_direct && G::staged)
@@ -301,34 +516,69 @@ __global__ void __launch_bounds__((geom<T, L, ROLE>::BS)) k_pow2(args<T> a)
         {
             const int e = tid + k * BS;
             if (e < lim)
-                a.out[base + e] = lds[e + (e >> 4)];
+                st_elem<T, PREC>(a.out, base + e, lds[e + (e >> 4)]);
         }
     }
     else if constexpr (ROLE == role_direct)
     {
         if (valid)
         {
-            cx<T>* dst = a.out + g * N + t;
+            const long long base = g * N + t;
 #pragma unroll
             for (int k = 0; k < E; ++k)
-                dst[k * TP] = v[k];
+                st_elem<T, PREC>(a.out, base + k * TP, v[k]);
         }
     }
     else if constexpr (ROLE == role_col)
     {
-        if (valid)
+        const long long bl            = g >> a.lo;
+        const long long col           = g & ((1ll << a.lo) - 1);
+        const long long base          = (bl << (a.lo + L)) + col;
+        const unsigned long long mask = (1ull << a.lbits) - 1;
+#pragma unroll
+        for (int k = 0; k < E; ++k)
         {
-            const long long bl            = g >> a.lo;
-            const long long col           = g & ((1ll << a.lo) - 1);
-            cx<T>* dst                    = a.out + (bl << (a.lo + L)) + col;
-            const unsigned long long mask = (1ull << a.lbits) - 1;
+            const int k2               = t + k * TP;
+            const unsigned long long m = (unsigned long long)col * (unsigned long long)k2; // < n
+            const cx<T> w              = cmul(a.ta[m >> a.lbits], a.tb[m & mask]);
+            v[k]                       = ctw<INV>(v[k], w);
+        }
+        if constexpr (PREC == prec_full)
+        {
+            if (valid)
+            {
+#pragma unroll
+                for (int k = 0; k < E; ++k)
+                    st_elem<T, PREC>(a.out, base + ((long long)(t + k * TP) << a.lo), v[k]);
+            }
+        }
+        else
+        {
+            // one scale for the workgroup's columns: wave maxima through LDS (free after the passes)
+            float mx = 0.f;
 #pragma unroll
             for (int k = 0; k < E; ++k)
+                mx = fmaxf(mx, fmaxf(fabsf((float)v[k].re), fabsf((float)v[k].im)));
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1)
+                mx = fmaxf(mx, __shfl_xor(mx, off));
+            float* red = reinterpret_cast<float*>(lds);
+            if (__lane_id() == 0)
+                red[tid >> 6] = mx;
+            lds_barrier();
+            float bm = 0.f;
+#pragma unroll
+            for (int w = 0; w < BS / 64; ++w)
+                bm = fmaxf(bm, red[w]);
+            const int sc = mid_scale_exp(bm);
+            if (tid == 0)
+                a.scales[blk] = sc;
+            if (valid)
             {
-                const int k2               = t + k * TP;
-                const unsigned long long m = (unsigned long long)col * (unsigned long long)k2; // < n
-                const cx<T> w              = cmul(a.ta[m >> a.lbits], a.tb[m & mask]);
-                dst[(long long)k2 << a.lo] = ctw<INV>(v[k], w);
+#pragma unroll
+                for (int k = 0; k < E; ++k)
+                    st_elem<T, prec_f16>(a.out, base + ((long long)(t + k * TP) << a.lo),
+                                         { ldexpf(v[k].re, sc), ldexpf(v[k].im, sc) });
             }
         }
     }
@@ -344,24 +594,24 @@ __global__ void __launch_bounds__((geom<T, L, ROLE>::BS)) k_pow2(args<T> a)
         const long long g2 = (long long)blk * W + tb2;
         if (g2 < a.count)
         {
-            const long long bl = g2 >> a.lo;
-            const long long k2 = g2 & ((1ll << a.lo) - 1);
-            cx<T>* dst         = a.out + (bl << (a.lo + L)) + k2;
+            const long long bl   = g2 >> a.lo;
+            const long long k2   = g2 & ((1ll << a.lo) - 1);
+            const long long base = (bl << (a.lo + L)) + k2;
 #pragma unroll
             for (int k = 0; k < E; ++k)
             {
-                const int i                = i0 + k * TP;
-                dst[(long long)i << a.lo] = lds[laddr<ROLE, W, G::LD>(tb2, i)];
+                const int i = i0 + k * TP;
+                st_elem<T, PREC>(a.out, base + ((long long)i << a.lo), lds[laddr<ROLE, W, G::LD>(tb2, i)]);
             }
         }
     }
 }
 
-template <typename T, int L, bool INV, int ROLE>
+template <typename T, int L, bool INV, int ROLE, int PREC>
 void launch(args<T> a, long long count, hipStream_t s)
 {
     using G                  = geom<T, L, ROLE>;
-    static const bool lds_ok = (allow_lds(k_pow2<T, L, INV, ROLE>, G::lds_bytes), true);
+    static const bool lds_ok = (allow_lds(k_pow2<T, L, INV, ROLE, PREC>, G::lds_bytes), true);
     (void)lds_ok;
     if (count <= 0)
         return;
@@ -371,30 +621,44 @@ void launch(args<T> a, long long count, hipStream_t s)
     if (grd > 0x7fffffffll)
         throw std::runtime_error("batch too large for one launch");
     a.nblk = (unsigned)nb;
-    hipLaunchKernelGGL((k_pow2<T, L, INV, ROLE>), dim3((unsigned)grd), dim3(G::BS), G::lds_bytes, s, a);
+    hipLaunchKernelGGL((k_pow2<T, L, INV, ROLE, PREC>), dim3((unsigned)grd), dim3(G::BS), G::lds_bytes, s, a);
     hip_check(hipGetLastError(), "power-of-two FFT launch");
 }
 
-template <typename T, bool INV, int ROLE, int LO, int HI>
+template <typename T, bool INV, int ROLE, int PREC, int LO, int HI>
 void launch_len(int L, const args<T>& a, long long count, hipStream_t s)
 {
     if constexpr (LO <= HI)
     {
         if (L == LO)
-            return launch<T, LO, INV, ROLE>(a, count, s);
-        return launch_len<T, INV, ROLE, LO + 1, HI>(L, a, count, s);
+            return launch<T, LO, INV, ROLE, PREC>(a, count, s);
+        return launch_len<T, INV, ROLE, PREC, LO + 1, HI>(L, a, count, s);
     }
     else
         throw std::logic_error("no power-of-two kernel for this length");
 }
 
-template <typename T, int ROLE, int LO, int HI>
+template <typename T, int ROLE, int PREC, int LO, int HI>
 void launch_any(bool inv, int L, const args<T>& a, long long count, hipStream_t s)
 {
     if (inv)
-        launch_len<T, true, ROLE, LO, HI>(L, a, count, s);
+        launch_len<T, true, ROLE, PREC, LO, HI>(L, a, count, s);
     else
-        launch_len<T, false, ROLE, LO, HI>(L, a, count, s);
+        launch_len<T, false, ROLE, PREC, LO, HI>(L, a, count, s);
+}
+
+// columns per workgroup of the column kernel of 2^L points (the row kernel's scale grouping)
+template <typename T, int LO, int HI>
+int col_width(int L)
+{
+    if constexpr (LO <= HI)
+    {
+        if (L == LO)
+            return geom<T, LO, role_col>::W;
+        return col_width<T, LO + 1, HI>(L);
+    }
+    else
+        return 1;
 }
 
 // pass tables of 2^L points (layout of pass_twoff), FP64-generated
@@ -427,13 +691,18 @@ template <typename T>
 struct pow2_fft<T>::impl
 {
     int L     = 0;
+    int prec  = p2::prec_full;
     bool four = false;
     int L1 = 0, L2 = 0; // four-step: rows of n1 = 2^L1 points, columns of n2 = 2^L2 points
     int lbits = 0;
     dev_mem tw;
     long long o_row = 0, o_ta = 0, o_tb = 0;
-    dev_mem scratch;
+    dev_mem scratch, scales;
+    int gshift      = 0;
     long long chunk = 1;
+
+    template <int PREC>
+    void run(bool inverse, const void* in, void* out, long long batch, hipStream_t s);
 };
 
 template <typename T>
@@ -444,13 +713,16 @@ bool pow2_fft<T>::supports(size_t n)
 }
 
 template <typename T>
-pow2_fft<T>::pow2_fft(size_t n, hipStream_t s) : p(new impl)
+pow2_fft<T>::pow2_fft(size_t n, hipStream_t s, int prec) : p(new impl)
 {
     const int L = p2::log2_exact(n);
     if (!supports(n))
         throw std::invalid_argument("pow2_fft: unsupported length");
+    if (prec != p2::prec_full && (sizeof(T) != 4 || (prec != p2::prec_bf16 && prec != p2::prec_f16)))
+        throw std::invalid_argument("pow2_fft: half-precision storage is computed in f32");
     const int le = p2::log_e<T>;
     p->L         = L;
+    p->prec      = prec;
     if (L <= p2::max_log<T>)
     {
         const int cnt = p2::pass_twoff(L, le, p2::pass_count(L, le));
@@ -475,57 +747,78 @@ pow2_fft<T>::pow2_fft(size_t n, hipStream_t s) : p(new impl)
     p2::gen_pass_tables<T>(p->L1, t + p->o_row, s);
     gen_twiddles(t + p->o_ta, nTA, (long long)n, nTB, 0, tw_linear, 0, s);
     gen_twiddles(t + p->o_tb, nTB, (long long)n, 1, 0, tw_linear, 0, s);
+    int w = p2::col_width<T, p2::min_split_log, p2::max_log<T>>(p->L2);
+    while ((1 << p->gshift) < w)
+        ++p->gshift;
     // rounds of at most 512 MiB of intermediate: few launches (measured faster than rounds small
     // enough to stay in the last-level cache), bounded scratch
     size_t cap = (size_t)512 << 20;
     if (const char* e = std::getenv("KFR_AMD_FOURSTEP_CHUNK_MB"))
         cap = (size_t)std::max(1, atoi(e)) << 20;
-    p->chunk = (long long)std::max<size_t>(1, cap / (n * sizeof(cx<T>)));
+    const size_t mid = prec == p2::prec_full ? sizeof(cx<T>) : 2 * sizeof(_Float16);
+    p->chunk         = (long long)std::max<size_t>(1, cap / (n * mid));
 }
 
 template <typename T>
 pow2_fft<T>::~pow2_fft() = default;
 
 template <typename T>
+template <int PREC>
+void pow2_fft<T>::impl::run(bool inverse, const void* in, void* out, long long batch, hipStream_t s)
+{
+    using p2::args;
+    constexpr int ML = p2::max_log<T>;
+    const size_t es  = PREC == p2::prec_full ? sizeof(cx<T>) : 2 * sizeof(_Float16); // HBM element size
+    const cx<T>* t   = tw.template as<cx<T>>();
+    if (!four)
+    {
+        args<T> a;
+        a.in  = in;
+        a.out = out;
+        a.tw  = t;
+        p2::launch_any<T, role_direct, PREC, 1, ML>(inverse, L, a, batch, s);
+        return;
+    }
+    const long long n = 1ll << L;
+    const size_t mid  = PREC == p2::prec_full ? sizeof(cx<T>) : 2 * sizeof(_Float16);
+    scratch.ensure((size_t)std::min(chunk, batch) * n * mid);
+    if (PREC != p2::prec_full)
+        scales.ensure((size_t)(std::min(chunk, batch) << L1 >> gshift) * sizeof(int));
+    for (long long b0 = 0; b0 < batch; b0 += chunk)
+    {
+        const long long nb = std::min(chunk, batch - b0);
+        args<T> a;
+        a.ta     = t + o_ta;
+        a.tb     = t + o_tb;
+        a.lbits  = lbits;
+        a.scales = scales.template as<int>();
+        a.gshift = gshift;
+        a.in     = static_cast<const char*>(in) + b0 * n * es;
+        a.out    = scratch.ptr;
+        a.tw     = t;
+        a.lo     = L1;
+        p2::launch_any<T, role_col, PREC, p2::min_split_log, ML>(inverse, L2, a, nb << L1, s);
+        a.in  = scratch.ptr;
+        a.out = static_cast<char*>(out) + b0 * n * es;
+        a.tw  = t + o_row;
+        a.lo  = L2;
+        p2::launch_any<T, role_row, PREC, p2::min_split_log, ML>(inverse, L1, a, nb << L2, s);
+    }
+}
+
+template <typename T>
 void pow2_fft<T>::run(bool inverse, const void* in, void* out, long long batch, hipStream_t s)
 {
     if (batch <= 0)
         return;
-    using p2::args;
-    constexpr int ML = p2::max_log<T>;
-    const cx<T>* x   = static_cast<const cx<T>*>(in);
-    cx<T>* y         = static_cast<cx<T>*>(out);
-    const cx<T>* t   = p->tw.template as<cx<T>>();
-    if (!p->four)
+    if constexpr (sizeof(T) == 4)
     {
-        args<T> a;
-        a.in  = x;
-        a.out = y;
-        a.tw  = t;
-        p2::launch_any<T, role_direct, 1, ML>(inverse, p->L, a, batch, s);
-        return;
+        if (p->prec == p2::prec_bf16)
+            return p->template run<p2::prec_bf16>(inverse, in, out, batch, s);
+        if (p->prec == p2::prec_f16)
+            return p->template run<p2::prec_f16>(inverse, in, out, batch, s);
     }
-    const long long n = 1ll << p->L;
-    p->scratch.ensure((size_t)std::min(p->chunk, batch) * n * sizeof(cx<T>));
-    cx<T>* z = p->scratch.template as<cx<T>>();
-    for (long long b0 = 0; b0 < batch; b0 += p->chunk)
-    {
-        const long long nb = std::min(p->chunk, batch - b0);
-        args<T> a;
-        a.ta    = t + p->o_ta;
-        a.tb    = t + p->o_tb;
-        a.lbits = p->lbits;
-        a.in    = x + b0 * n;
-        a.out   = z;
-        a.tw    = t;
-        a.lo    = p->L1;
-        p2::launch_any<T, role_col, p2::min_split_log, ML>(inverse, p->L2, a, nb << p->L1, s);
-        a.in  = z;
-        a.out = y + b0 * n;
-        a.tw  = t + p->o_row;
-        a.lo  = p->L2;
-        p2::launch_any<T, role_row, p2::min_split_log, ML>(inverse, p->L1, a, nb << p->L2, s);
-    }
+    p->template run<p2::prec_full>(inverse, in, out, batch, s);
 }
 
 template <typename T>
@@ -539,11 +832,16 @@ std::string pow2_fft<T>::describe() const
             r += (q ? "x" : "") + std::to_string(1 << p2::pass_lr(L, le, q));
         return r;
     };
+    const std::string pr = p->prec == p2::prec_bf16 ? " storage=bf16 radix16=mfma"
+                           : p->prec == p2::prec_f16 ? " storage=f16 radix16=mfma"
+                                                      : "";
     if (!p->four)
         return "pow2 radices=" + radices(p->L) + " threads/transform=" +
-               std::to_string((1 << p->L) >> p2::pass_le(p->L, le));
+               std::to_string((1 << p->L) >> p2::pass_le(p->L, le)) + pr;
     return "pow2 four-step n1=" + std::to_string(1ll << p->L1) + " n2=" + std::to_string(1ll << p->L2) +
-           " radices=" + radices(p->L2) + "|" + radices(p->L1) + " chunk=" + std::to_string(p->chunk);
+           " radices=" + radices(p->L2) + "|" + radices(p->L1) + " chunk=" + std::to_string(p->chunk) + pr +
+           (p->prec != p2::prec_full ? " intermediate=f16 scaled per " + std::to_string(1 << p->gshift) + " columns"
+                                     : "");
 }
 
 template class pow2_fft<float>;

@@ -18,7 +18,9 @@ public:
     // true for N = 2^L that one kernel (L <= 14 for f32, 13 for f64) or the two-kernel four-step
     // split (up to twice that) covers
     static bool supports(size_t n);
-    pow2_fft(size_t n, hipStream_t s);
+    // prec: storage of the data in HBM, 0 = complex T, 1 = bfloat16 pairs, 2 = binary16 pairs (T = float,
+    // radix-16 passes on MFMA)
+    pow2_fft(size_t n, hipStream_t s, int prec = 0);
     ~pow2_fft();
     // batch transforms: in and out hold batch*n complex values each, item b at offset b*n; in == out
     // is allowed. No scaling (the inverse is unnormalised).

@@ -390,6 +390,36 @@ class dft_batch:
             self._p = None
 
 
+class dft_plan_half:
+    """Complex DFT of a power-of-two size on half-precision data (kfr_amd_dft_half_*): pairs of
+    bfloat16 (storage "bf16") or binary16 ("f16"), computed in FP32 with the radix-16 passes on MFMA.
+    Buffers are torch tensors (torch.bfloat16 / torch.float16, interleaved re, im) or any array of
+    4-byte complex values. No counterpart in the reference."""
+
+    def __init__(self, size: int, storage: str = "bf16"):
+        self.storage = storage
+        code = {"bf16": 1, "f16": 2}[storage]
+        self._h = lib().kfr_amd_dft_half_create(code, size)
+        check(bool(self._h))
+        self.size = int(size)
+
+    def execute_batch(self, out, in_, batch: int, inverse: bool = False, stream=None):
+        s = _stream_ptr(stream) if not isinstance(out, np.ndarray) else None
+        check(lib().kfr_amd_dft_half_execute_batch(self._h, _ptr(out), _ptr(in_), batch, int(inverse), s) == 0)
+        return out
+
+    def describe(self) -> str:
+        buf = C.create_string_buffer(1024)
+        check(lib().kfr_amd_dft_half_describe(self._h, buf, len(buf)) >= 0)
+        return buf.value.decode()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().kfr_amd_dft_half_delete(h)
+            self._h = None
+
+
 class dft_cache:
     """Process-wide cache of complex plans (include/kfr/dft/cache.hpp:44-126). The key includes the
     element type and the size, as the reference's does."""
