@@ -351,7 +351,7 @@ def main() -> int:
     fs = [p for p in per if p["kind"] == "c2c" and p["dtype"] == "f32" and p["n"] > (1 << 14)]
     lds_t = sum(p["ms"] for p in lds)
     lds_bytes = sum(alg_bytes("c2c", "f32", p["n"], p["batch"]) for p in lds)
-    achieved = lds_bytes / (lds_t * 1e-3) * 1e-9
+    achieved = lds_bytes / (lds_t * 1e-3) * 1e-9 if lds_t else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": "k_fft<float,false,role_direct> (complex-f32 N=2^4..2^14, one launch per size, "

@@ -11,6 +11,7 @@
 
 #include "fft_device.hpp"
 #include "fft_engine.hpp"
+#include "fft_mixed.hpp"
 #include "fft_pow2.hpp"
 
 #include <algorithm>
@@ -758,8 +759,10 @@ struct core_fft
     dev_mem tw;
     long long twA = 0, twB = 0, twTA = 0, twTB = 0;
     dev_mem scratch;
-    // power-of-two lengths: specialised kernels for contiguous batches (plain loads and stores)
+    // power-of-two lengths and mixed lengths built from the menu: specialised kernels for contiguous
+    // batches (plain loads and stores)
     std::unique_ptr<pow2_fft<T>> pow2;
+    std::unique_ptr<mixed_fft<T>> mixed;
 
     static bool fits_lds(size_t len)
     {
@@ -785,6 +788,8 @@ struct core_fft
         n = len;
         if (pow2_fft<T>::supports(n))
             pow2.reset(new pow2_fft<T>(n, s));
+        else if (mixed_fft<T>::supports(n))
+            mixed.reset(new mixed_fft<T>(n, s));
         if (fits_lds(n))
         {
             make_stockham((long long)n, S, pmax_of<T>);
@@ -880,10 +885,14 @@ struct core_fft
     {
         if (batch <= 0)
             return;
-        if (pow2 && plain_batch(ld) && plain_batch(st))
+        if ((pow2 || mixed) && plain_batch(ld) && plain_batch(st))
         {
-            pow2->run(INV, static_cast<const cx<T>*>(ld.p) + ld.b0 * (long long)n,
-                      static_cast<cx<T>*>(st.p) + st.b0 * (long long)n, batch, s);
+            const cx<T>* x = static_cast<const cx<T>*>(ld.p) + ld.b0 * (long long)n;
+            cx<T>* y       = static_cast<cx<T>*>(st.p) + st.b0 * (long long)n;
+            if (pow2)
+                pow2->run(INV, x, y, batch, s);
+            else
+                mixed->run(INV, x, y, batch, s);
             return;
         }
         const cx<T>* t = tw.as<cx<T>>();

@@ -194,6 +194,8 @@ struct gpu_dft<T>::impl
             }
             if (core->pow2)
                 info.contiguous = core->pow2->describe();
+            else if (core->mixed)
+                info.contiguous = core->mixed->describe();
         }
         else
             init_bluestein(s);
