@@ -108,7 +108,7 @@ struct geom
 template <int ROLE, int W, int LD>
 __device__ __forceinline__ int laddr(int tb, int i)
 {
-    const int x = ROLE == role_col ? i * W + tb : tb * LD + i;
+    const int x = (ROLE == role_col || ROLE == role_pass) ? i * W + tb : tb * LD + i;
     return x + (x >> 4);
 }
 
@@ -147,6 +147,7 @@ struct args
     int lo           = 0;       // four-step: log2 of the other dimension (col: n1, row: n2)
     int lbits        = 0;
     int gshift       = 0; // half-precision row kernel: log2 of the column kernel's columns per workgroup
+    int ns_log       = 0; // multi-pass: log2 of the stride Ns of this pass (twiddles W_{Ns*R} from ta, tb)
     unsigned nblk    = 0; // workgroups with work (the grid of split kernels is a multiple of 8)
 };
 
@@ -435,8 +436,9 @@ __global__ void __launch_bounds__((geom<T, L, ROLE>::BS)) k_pow2(args<T> a)
             return; // whole workgroup, before any barrier
     }
     const int tid     = threadIdx.x;
-    const int tb      = ROLE == role_col ? tid % W : tid / TP;
-    const int t       = ROLE == role_col ? tid / W : tid % TP;
+    constexpr bool COLS = ROLE == role_col || ROLE == role_pass; // lanes on consecutive columns
+    const int tb      = COLS ? tid % W : tid / TP;
+    const int t       = COLS ? tid / W : tid % TP;
     const long long g = (long long)blk * W + tb;
     const bool valid  = g < a.count;
     cx<T> v[E];
@@ -463,7 +465,7 @@ __global__ void __launch_bounds__((geom<T, L, ROLE>::BS)) k_pow2(args<T> a)
             v[k] = lds[laddr<ROLE, W, G::LD>(tb, t + k * TP)];
         lds_barrier();
     }
-    else if constexpr (ROLE == role_col)
+    else if constexpr (ROLE == role_col || ROLE == role_pass)
     {
         const long long bl   = g >> a.lo;
         const long long col  = g & ((1ll << a.lo) - 1);
@@ -471,6 +473,21 @@ __global__ void __launch_bounds__((geom<T, L, ROLE>::BS)) k_pow2(args<T> a)
 #pragma unroll
         for (int k = 0; k < E; ++k)
             v[k] = valid ? ld_elem<T, PREC>(a.in, base + ((long long)(t + k * TP) << a.lo)) : cx<T>{ T(0), T(0) };
+        if constexpr (ROLE == role_pass)
+        {
+            if (a.ns_log > 0)
+            {
+                // input r of butterfly j is multiplied by W_{Ns*R}^(r * (j mod Ns))
+                const unsigned long long kt   = (unsigned long long)col & ((1ull << a.ns_log) - 1);
+                const unsigned long long mask = (1ull << a.lbits) - 1;
+#pragma unroll
+                for (int k = 0; k < E; ++k)
+                {
+                    const unsigned long long m = kt * (unsigned long long)(t + k * TP); // < Ns*R
+                    v[k]                       = ctw<INV>(v[k], cmul(a.ta[m >> a.lbits], a.tb[m & mask]));
+                }
+            }
+        }
     }
     else if constexpr (ROLE == role_row && PREC != prec_full)
     {
@@ -527,6 +544,40 @@ __global__ void __launch_bounds__((geom<T, L, ROLE>::BS)) k_pow2(args<T> a)
 #pragma unroll
             for (int k = 0; k < E; ++k)
                 st_elem<T, PREC>(a.out, base + k * TP, v[k]);
+        }
+    }
+    else if constexpr (ROLE == role_pass)
+    {
+        const long long bl  = g >> a.lo;
+        const long long col = g & ((1ll << a.lo) - 1);
+        if (a.ns_log == 0)
+        {
+            // first pass: output r of butterfly j goes to j*R + r, so the workgroup's W butterflies
+            // fill one contiguous run of W*R values; write it through LDS in order
+#pragma unroll
+            for (int k = 0; k < E; ++k)
+                lds[laddr<ROLE, W, G::LD>(tb, t + k * TP)] = v[k];
+            lds_barrier();
+            const long long g0   = (long long)blk * W;
+            const long long b0   = g0 >> a.lo;
+            const long long run  = (b0 << (a.lo + L)) + ((g0 & ((1ll << a.lo) - 1)) << L);
+            const long long lim  = (a.count - g0) << L;
+#pragma unroll
+            for (int k = 0; k < E; ++k)
+            {
+                const int q = tid + k * BS;
+                if (q < lim)
+                    st_elem<T, PREC>(a.out, run + q, lds[laddr<ROLE, W, G::LD>(q >> L, q & (N - 1))]);
+            }
+        }
+        else if (valid)
+        {
+            // output r of butterfly j goes to (j - j mod Ns)*R + j mod Ns + r*Ns
+            const long long kt   = col & ((1ll << a.ns_log) - 1);
+            const long long base = (bl << (a.lo + L)) + ((col >> a.ns_log) << (a.ns_log + L)) + kt;
+#pragma unroll
+            for (int k = 0; k < E; ++k)
+                st_elem<T, PREC>(a.out, base + ((long long)(t + k * TP) << a.ns_log), v[k]);
         }
     }
     else if constexpr (ROLE == role_col)
@@ -700,6 +751,12 @@ struct pow2_fft<T>::impl
     dev_mem scratch, scales;
     int gshift      = 0;
     long long chunk = 1;
+    // multi-pass Stockham over HBM (long lengths): log2 radix of every pass, its sub-DFT tables and
+    // its W_{Ns*R} two-level tables
+    bool multi = false;
+    std::vector<int> mp_log, mp_lbits;
+    std::vector<long long> mp_tw, mp_ta, mp_tb;
+    dev_mem scratch2;
 
     template <int PREC>
     void run(bool inverse, const void* in, void* out, long long batch, hipStream_t s);
@@ -728,6 +785,53 @@ pow2_fft<T>::pow2_fft(size_t n, hipStream_t s, int prec) : p(new impl)
         const int cnt = p2::pass_twoff(L, le, p2::pass_count(L, le));
         p->tw.reset((size_t)std::max(1, cnt) * sizeof(cx<T>));
         p2::gen_pass_tables<T>(L, p->tw.template as<cx<T>>(), s);
+        return;
+    }
+    // the longest lengths (measured: from 2^24 for f32, 2^23 for f64): HBM passes of radix 2^5..2^8 with
+    // 16 butterflies per workgroup (128-byte runs in and out, small workgroups); shorter lengths: the
+    // two-kernel four-step, whose extra pass costs more than its wider workgroups
+    int mp_min = sizeof(T) == 4 ? 24 : 23;
+    if (const char* e = std::getenv("KFR_AMD_MULTIPASS_MIN_LOG"))
+        mp_min = atoi(e);
+    if (prec == p2::prec_full && L >= mp_min)
+    {
+        p->multi     = true;
+        const int np = (L + 7) / 8;
+        for (int q = 0; q < np; ++q)
+            p->mp_log.push_back(L / np + (q < L % np ? 1 : 0));
+        long long off = 0;
+        int nsl       = 0;
+        for (int q = 0; q < np; ++q)
+        {
+            const int l = p->mp_log[q];
+            p->mp_tw.push_back(off);
+            off += (p2::pass_twoff(l, le, p2::pass_count(l, le)) + 1) / 2 * 2;
+            const int s  = nsl + l; // the pass's twiddles are powers of W_{2^s}
+            const int lb = (s + 1) / 2;
+            p->mp_lbits.push_back(lb);
+            p->mp_ta.push_back(off);
+            off += ((1ll << s >> lb) + 1 + 1) / 2 * 2;
+            p->mp_tb.push_back(off);
+            off += ((1ll << lb) + 1) / 2 * 2;
+            nsl += l;
+        }
+        p->tw.reset((size_t)off * sizeof(cx<T>));
+        cx<T>* t = p->tw.template as<cx<T>>();
+        nsl      = 0;
+        for (int q = 0; q < np; ++q)
+        {
+            const int l       = p->mp_log[q];
+            p2::gen_pass_tables<T>(l, t + p->mp_tw[q], s);
+            const long long M = 1ll << (nsl + l);
+            const int lb      = p->mp_lbits[q];
+            gen_twiddles(t + p->mp_ta[q], (M >> lb) + 1, M, 1ll << lb, 0, tw_linear, 0, s);
+            gen_twiddles(t + p->mp_tb[q], 1ll << lb, M, 1, 0, tw_linear, 0, s);
+            nsl += l;
+        }
+        size_t cap = (size_t)512 << 20;
+        if (const char* e = std::getenv("KFR_AMD_FOURSTEP_CHUNK_MB"))
+            cap = (size_t)std::max(1, atoi(e)) << 20;
+        p->chunk = (long long)std::max<size_t>(1, cap / (n * sizeof(cx<T>)));
         return;
     }
     p->four              = true;
@@ -770,6 +874,40 @@ void pow2_fft<T>::impl::run(bool inverse, const void* in, void* out, long long b
     constexpr int ML = p2::max_log<T>;
     const size_t es  = PREC == p2::prec_full ? sizeof(cx<T>) : 2 * sizeof(_Float16); // HBM element size
     const cx<T>* t   = tw.template as<cx<T>>();
+    if constexpr (PREC == p2::prec_full)
+    {
+        if (multi)
+        {
+            const long long n = 1ll << L;
+            const size_t bytes = (size_t)std::min(chunk, batch) * n * sizeof(cx<T>);
+            scratch.ensure(bytes);
+            if (mp_log.size() > 2)
+                scratch2.ensure(bytes);
+            void* buf[2] = { scratch.ptr, scratch2.ptr };
+            const int np = (int)mp_log.size();
+            for (long long b0 = 0; b0 < batch; b0 += chunk)
+            {
+                const long long nb = std::min(chunk, batch - b0);
+                int nsl            = 0;
+                for (int q = 0; q < np; ++q)
+                {
+                    const int l = mp_log[q];
+                    args<T> a;
+                    a.in     = q == 0 ? static_cast<const char*>(in) + b0 * n * es : buf[(q - 1) & 1];
+                    a.out    = q == np - 1 ? static_cast<char*>(out) + b0 * n * es : buf[q & 1];
+                    a.tw     = t + mp_tw[q];
+                    a.ta     = t + mp_ta[q];
+                    a.tb     = t + mp_tb[q];
+                    a.lbits  = mp_lbits[q];
+                    a.lo     = L - l;
+                    a.ns_log = nsl;
+                    p2::launch_any<T, role_pass, PREC, 5, 8>(inverse, l, a, nb << (L - l), s);
+                    nsl += l;
+                }
+            }
+            return;
+        }
+    }
     if (!four)
     {
         args<T> a;
@@ -835,6 +973,14 @@ std::string pow2_fft<T>::describe() const
     const std::string pr = p->prec == p2::prec_bf16 ? " storage=bf16 radix16=mfma"
                            : p->prec == p2::prec_f16 ? " storage=f16 radix16=mfma"
                                                       : "";
+    if (p->multi)
+    {
+        std::string r;
+        for (size_t q = 0; q < p->mp_log.size(); ++q)
+            r += (q ? "x" : "") + std::to_string(1 << p->mp_log[q]);
+        return "pow2 multi-pass radices=" + r + " (16 butterflies per workgroup, one HBM pass each) chunk=" +
+               std::to_string(p->chunk);
+    }
     if (!p->four)
         return "pow2 radices=" + radices(p->L) + " threads/transform=" +
                std::to_string((1 << p->L) >> p2::pass_le(p->L, le)) + pr;

@@ -1,5 +1,6 @@
-"""Power-of-two C2C kernels (one workgroup per transform group, and the four-step split) over
-contiguous device batches: every length 2^1..2^22, batch tails, in-place, both directions (gpu)."""
+"""Power-of-two C2C kernels (one workgroup per transform group, the four-step split, and the
+multi-pass Stockham over HBM for the longest lengths) over contiguous device batches: every length
+2^1..2^24, batch tails, in-place, both directions (gpu)."""
 from __future__ import annotations
 
 import numpy as np
@@ -14,10 +15,10 @@ pytestmark = pytest.mark.gpu
 
 def _batch_for(L):
     # odd batch sizes leave a partial last workgroup; about 2^20 values per case
-    return max(1, (1 << 20) >> L) + 1 if L <= 18 else 2
+    return max(1, (1 << 20) >> L) + 1 if L <= 18 else (2 if L <= 22 else 1)
 
 
-@pytest.mark.parametrize("L", list(range(1, 23)))
+@pytest.mark.parametrize("L", list(range(1, 25)))
 @pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
 def test_pow2_batch(L, dtype):
     torch = pytest.importorskip("torch")
