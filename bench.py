@@ -23,6 +23,8 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
+# HBM bytes per dispatch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this benchmark
+TRAFFIC_TABLE = "bench_sweep_hbm_traffic.md"
 
 
 def sweep_configs(quick: bool):
@@ -227,6 +229,7 @@ def main() -> int:
     import torch
 
     import kfr_amd
+    from kfr_amd.rocprof_summary import read_traffic
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
@@ -292,15 +295,18 @@ def main() -> int:
     t0 = time.perf_counter()
     start = torch.cuda.Event(enable_timing=True)
     end = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    start.record(stream)
-    for s in range(args.steps):
-        for i, c in enumerate(cfgs):
-            ev[s][i][0].record(stream)
-            run_cfg(c)
-            ev[s][i][1].record(stream)
-    end.record(stream)
-    torch.cuda.synchronize()
+    # every kernel launch of the timed region is also bracketed by HIP events on the same stream
+    # (recorded by the library), giving each kernel's average launch duration
+    with kfr_amd.kernel_timing() as kt:
+        torch.cuda.synchronize()
+        start.record(stream)
+        for s in range(args.steps):
+            for i, c in enumerate(cfgs):
+                ev[s][i][0].record(stream)
+                run_cfg(c)
+                ev[s][i][1].record(stream)
+        end.record(stream)
+        torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     total_ms = start.elapsed_time(end)
     ms_per_step = total_ms / args.steps
@@ -345,19 +351,32 @@ def main() -> int:
         k = f"{kind}_{dt}"
         errs[k] = max(errs.get(k, 0.0), e)
 
-    # dominant kernel: the single-launch LDS kernel of the complex-f32 sizes N <= 2^14 (one launch per
-    # size, 2 GiB algorithmic per launch) or the two four-step kernels, whichever takes longer
-    lds = [p for p in per if p["kind"] == "c2c" and p["dtype"] == "f32" and p["n"] <= (1 << 14)]
+    # dominant kernel: the kernel (template instance) with the largest total time in the timed region;
+    # achieved = its algorithmic bytes per launch / its average launch duration
+    kernels = kt.by_kernel()
+    top = []
+    for name, k in list(kernels.items())[:8]:
+        avg_ms = k["ms"] / k["launches"]
+        gbs = k["bytes"] / (k["ms"] * 1e-3) * 1e-9 if k["ms"] > 0 else 0.0
+        top.append({"kernel": name, "launches_per_step": k["launches"] / args.steps, "avg_us": round(avg_ms * 1e3, 1),
+                    "alg_bytes_per_launch": k["bytes"] // k["launches"], "gbs": round(gbs, 1),
+                    "frac": round(gbs / HBM_PEAK_GBS, 4), "share_of_step": round(k["ms"] / total_ms, 4)})
+    roofline = None
+    if top:
+        d = top[0]
+        tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", TRAFFIC_TABLE)
+        tr = read_traffic(tfile, d["kernel"])
+        roofline = {"bound": "hbm", "achieved": d["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["frac"],
+                    "traffic": int(tr) if tr is not None else None,
+                    "kernel": d["kernel"], "avg_us": d["avg_us"], "alg_bytes_per_launch": d["alg_bytes_per_launch"],
+                    "launches_per_step": d["launches_per_step"], "share_of_step": d["share_of_step"],
+                    "traffic_source": f"profiles/{TRAFFIC_TABLE}" if tr is not None else None}
     fs = [p for p in per if p["kind"] == "c2c" and p["dtype"] == "f32" and p["n"] > (1 << 14)]
-    lds_t = sum(p["ms"] for p in lds)
-    lds_bytes = sum(alg_bytes("c2c", "f32", p["n"], p["batch"]) for p in lds)
-    achieved = lds_bytes / (lds_t * 1e-3) * 1e-9 if lds_t else 0.0
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_fft<float,false,role_direct> (complex-f32 N=2^4..2^14, one launch per size, "
-                          "2 GiB algorithmic bytes per launch)"}
     fs_t = sum(p["ms"] for p in fs)
     fs_bytes = sum(alg_bytes("c2c", "f32", p["n"], p["batch"]) for p in fs)
+    lds = [p for p in per if p["kind"] == "c2c" and p["dtype"] == "f32" and p["n"] <= (1 << 14)]
+    lds_t = sum(p["ms"] for p in lds)
+    lds_bytes = sum(alg_bytes("c2c", "f32", p["n"], p["batch"]) for p in lds)
 
     uneven = None
     if not args.no_uneven and not args.only:
@@ -392,7 +411,9 @@ def main() -> int:
         "rel_rms_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs.items()},
         "uneven_batch": uneven,
         "precision": precision,
-        "fourstep_c2c_f32_frac": round(fs_bytes / (fs_t * 1e-3) * 1e-9 / HBM_PEAK_GBS, 4) if fs_t else None,
+        "top_kernels": top,
+        "single_kernel_c2c_f32_frac": round(lds_bytes / (lds_t * 1e-3) * 1e-9 / HBM_PEAK_GBS, 4) if lds_t else None,
+        "long_c2c_f32_frac": round(fs_bytes / (fs_t * 1e-3) * 1e-9 / HBM_PEAK_GBS, 4) if fs_t else None,
         "wall_s": round(wall, 3),
         "per_config": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in p.items()} for p in per],
     }

@@ -282,6 +282,18 @@ KFR_AMD_API void kfr_amd_inject_alloc_failure(int n);
 /* Synchronise the library's stream. */
 KFR_AMD_API int kfr_amd_synchronize(void);
 
+/* Kernel timing. Between kfr_amd_timing_begin and kfr_amd_timing_end, every launch of a transform
+ * kernel (power-of-two, mixed-radix, generic LDS, Bluestein and real-transform kernels) is bracketed by
+ * two HIP events recorded on the stream it is launched on. kfr_amd_timing_end stops recording, waits
+ * for those events and returns the number of launches recorded (-1 on error); kfr_amd_timing_get then
+ * reads launch `index`: the kernel's template instance as profilers spell it (e.g.
+ * "k_pow2<float, 12, false, 2, 0>"), its algorithmic bytes (one read of the values it transforms and
+ * one write of its results, tables excluded) and its duration in milliseconds. Returns 0, or -1 for an
+ * index out of range. No counterpart in the reference: a measurement aid. */
+KFR_AMD_API int kfr_amd_timing_begin(void);
+KFR_AMD_API long long kfr_amd_timing_end(void);
+KFR_AMD_API int kfr_amd_timing_get(long long index, char* name, size_t namelen, long long* bytes, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

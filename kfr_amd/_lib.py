@@ -97,6 +97,9 @@ def _proto(lib):
     f("kfr_amd_dft_half_delete", None, vp)
     f("kfr_amd_inject_alloc_failure", None, C.c_int)
     f("kfr_amd_synchronize", C.c_int)
+    f("kfr_amd_timing_begin", C.c_int)
+    f("kfr_amd_timing_end", C.c_longlong)
+    f("kfr_amd_timing_get", C.c_int, C.c_longlong, C.c_char_p, c_size, vp, vp)
 
 
 def lib():

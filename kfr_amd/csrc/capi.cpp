@@ -5,6 +5,7 @@
 #include "fft_engine.hpp"
 #include "fft_md.hpp"
 #include "fft_pow2.hpp"
+#include "fft_timing.hpp"
 #include "plan_desc.hpp"
 
 #include <hip/hip_runtime.h>
@@ -17,6 +18,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <vector>
 
 using namespace kfr_amd;
 
@@ -32,6 +34,18 @@ void set_error(const std::string& s)
     const size_t n = std::min(s.size(), latest_error.size() - 1);
     std::copy_n(s.begin(), n, latest_error.begin());
     std::fill(latest_error.begin() + n, latest_error.end(), 0);
+}
+
+struct timing_result
+{
+    std::string name;
+    long long bytes;
+    double ms;
+};
+std::vector<timing_result>& timing_results()
+{
+    static std::vector<timing_result> r;
+    return r;
 }
 
 template <typename Fn, typename R>
@@ -1172,6 +1186,81 @@ extern "C"
     }
 
     void kfr_amd_inject_alloc_failure(int n) { set_alloc_failure_countdown(n); }
+
+    int kfr_amd_timing_begin(void)
+    {
+        return try_fn(
+            []
+            {
+                timing::state& st = timing::get();
+                std::lock_guard<std::mutex> g(st.m);
+                for (auto& r : st.recs)
+                {
+                    st.pool.push_back(r.start);
+                    st.pool.push_back(r.stop);
+                }
+                st.recs.clear();
+                timing_results().clear();
+                st.on = true;
+                return 0;
+            },
+            -1);
+    }
+
+    long long kfr_amd_timing_end(void)
+    {
+        return try_fn(
+            []
+            {
+                timing::state& st = timing::get();
+                st.on = false;
+                std::lock_guard<std::mutex> g(st.m);
+                auto& out = timing_results();
+                out.clear();
+                std::string err;
+                for (auto& r : st.recs)
+                {
+                    float ms      = 0.f;
+                    hipError_t e  = hipEventSynchronize(r.stop);
+                    if (e == hipSuccess)
+                        e = hipEventElapsedTime(&ms, r.start, r.stop);
+                    if (e != hipSuccess && err.empty())
+                        err = hipGetErrorString(e);
+                    out.push_back(timing_result{ r.name, r.bytes, (double)ms });
+                    st.pool.push_back(r.start);
+                    st.pool.push_back(r.stop);
+                }
+                st.recs.clear();
+                if (!err.empty())
+                    throw std::runtime_error("kernel timing: " + err);
+                return (long long)out.size();
+            },
+            -1ll);
+    }
+
+    int kfr_amd_timing_get(long long index, char* name, size_t namelen, long long* bytes, double* ms)
+    {
+        return try_fn(
+            [&]
+            {
+                const auto& out = timing_results();
+                if (index < 0 || index >= (long long)out.size())
+                    throw std::out_of_range("timing record index out of range");
+                const timing_result& r = out[(size_t)index];
+                if (name && namelen)
+                {
+                    const size_t n = std::min(namelen - 1, r.name.size());
+                    std::memcpy(name, r.name.data(), n);
+                    name[n] = 0;
+                }
+                if (bytes)
+                    *bytes = r.bytes;
+                if (ms)
+                    *ms = r.ms;
+                return 0;
+            },
+            -1);
+    }
 
     int kfr_amd_synchronize(void)
     {

@@ -13,6 +13,7 @@
 #include "fft_engine.hpp"
 #include "fft_mixed.hpp"
 #include "fft_pow2.hpp"
+#include "fft_timing.hpp"
 
 #include <algorithm>
 #include <climits>
@@ -731,6 +732,9 @@ void launch_fft(bool mixed, const lds_geom& G, const cx<T>* tw, long long batch,
                 const xstore<T>& st, const fourstep_args<T>& fs, unsigned grid, unsigned block, size_t sh,
                 hipStream_t s)
 {
+    // complex values in and out: batch items (columns or rows in the four-step roles) of G.n points
+    timing::span ts([mixed] { return timing::kernel_name("k_fft", timing::type_name<T>(), INV, ROLE, mixed); },
+                    2 * (long long)sizeof(cx<T>) * batch * G.n, s);
     if (mixed)
     {
         allow_lds(k_fft<T, INV, ROLE, true>, sh);

@@ -258,6 +258,8 @@ struct gpu_dft<T>::impl
             lds_geom G      = make_geom(bsS, bs_tpb, 0, 1, (int)M);
             const size_t sh = lds_padded_elems((size_t)bs_tpb * M) * sizeof(cx<T>);
             allow_lds(k_bluestein<T, INV>, sh);
+            timing::span ts([] { return timing::kernel_name("k_bluestein", timing::type_name<T>(), INV); },
+                            2 * (long long)sizeof(cx<T>) * batch * (long long)n, s);
             hipLaunchKernelGGL((k_bluestein<T, INV>), dim3(nblocks(batch, bs_tpb)), dim3(bsS.tpt * bs_tpb), sh, s, G,
                                bs_tw.as<cx<T>>(), batch, H.as<cx<T>>(), lc, sc, make_fastdiv((uint32_t)M));
             hip_check(hipGetLastError(), "bluestein launch");
@@ -444,7 +446,10 @@ struct gpu_dft_real<T>::impl
             const size_t sh = lds_padded_elems((size_t)f.tpb * csize) * sizeof(cx<T>);
             const dim3 grid(nblocks(batch, f.tpb)), block(f.S.tpt * f.tpb);
             const fastdiv nd = make_fastdiv((uint32_t)csize);
-            if (is_mixed(f.S))
+            const bool mx    = is_mixed(f.S);
+            timing::span ts([mx] { return timing::kernel_name("k_real_fwd", timing::type_name<T>(), mx); },
+                            2 * (long long)sizeof(cx<T>) * batch * (long long)csize, s);
+            if (mx)
             {
                 allow_lds(k_real_fwd<T, true>, sh);
                 hipLaunchKernelGGL((k_real_fwd<T, true>), grid, block, sh, s, G, f.tw.template as<cx<T>>(), batch, ld,
@@ -471,6 +476,8 @@ struct gpu_dft_real<T>::impl
             const long long count = (long long)(csize + 1) / 2;
             dim3 grid(nblocks(count + 1, 256), (unsigned)std::min<long long>(nb, 65535),
                       (unsigned)((nb + 65534) / 65535));
+            timing::span ts([] { return timing::kernel_name("k_real_repack", timing::type_name<T>()); },
+                            2 * (long long)sizeof(cx<T>) * nb * per, s);
             hipLaunchKernelGGL(k_real_repack<T>, grid, dim3(256), 0, s, out + b0 * odist, odist, Z.as<cx<T>>(), per,
                                rtw.as<cx<T>>(), (long long)csize, fmt, nb);
             hip_check(hipGetLastError(), "real repack launch");

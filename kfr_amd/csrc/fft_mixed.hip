@@ -9,6 +9,7 @@
 // menu: the column kernel transforms W adjacent columns per workgroup and applies W_n^(c*k2), the row
 // kernel stores its rows transposed through LDS; the other dimension is a run-time divisor (fastdiv).
 #include "fft_mixed.hpp"
+#include "fft_timing.hpp"
 
 #include "fft_core.hpp"
 
@@ -315,6 +316,8 @@ void mlaunch(margs a, long long count, hipStream_t s)
     if (grd > 0x7fffffffll)
         throw std::runtime_error("batch too large for one launch");
     a.nblk = (unsigned)nb;
+    timing::span ts([] { return timing::kernel_name("k_mix", N, INV, ROLE); },
+                    2 * (long long)sizeof(cx<float>) * count * N, s);
     hipLaunchKernelGGL((k_mix<N, INV, ROLE>), dim3((unsigned)grd), dim3(G::BS), G::lds_bytes, s, a);
     hip_check(hipGetLastError(), "mixed-radix FFT launch");
 }

@@ -15,6 +15,7 @@
 // W-element runs. Workgroups of both kernels are ordered so that neighbouring column (or row)
 // groups land on the same XCD and share its L2.
 #include "fft_pow2.hpp"
+#include "fft_timing.hpp"
 
 #include "fft_core.hpp"
 
@@ -671,7 +672,10 @@ void launch(args<T> a, long long count, hipStream_t s)
     const long long grd = ROLE == role_direct ? nb : (nb + 7) / 8 * 8;
     if (grd > 0x7fffffffll)
         throw std::runtime_error("batch too large for one launch");
-    a.nblk = (unsigned)nb;
+    a.nblk            = (unsigned)nb;
+    const long long es = PREC == prec_full ? (long long)sizeof(cx<T>) : 4;
+    timing::span ts([] { return timing::kernel_name("k_pow2", timing::type_name<T>(), L, INV, ROLE, PREC); },
+                    2 * es * (count << L), s);
     hipLaunchKernelGGL((k_pow2<T, L, INV, ROLE, PREC>), dim3((unsigned)grd), dim3(G::BS), G::lds_bytes, s, a);
     hip_check(hipGetLastError(), "power-of-two FFT launch");
 }
