@@ -14,6 +14,15 @@
 // transforms the n1-point rows and stores them transposed through LDS, so both kernels move
 // W-element runs. Workgroups of both kernels are ordered so that neighbouring column (or row)
 // groups land on the same XCD and share its L2.
+//
+// The longest lengths run as a multi-pass Stockham over HBM instead: passes of radix 2^5..2^8, each
+// one kernel ("pass") in which a workgroup transforms 16 butterflies of stride Ns (consecutive lanes
+// on consecutive butterflies), with input twiddles W_{Ns*R}^(r*(j mod Ns)) from a two-level table,
+// and writes outputs (j - j mod Ns)*R + j mod Ns + r*Ns; the first pass (Ns = 1) writes its
+// contiguous output runs through LDS.
+//
+// Half-precision storage (bfloat16 or binary16 pairs, T = float) reuses the same kernels with
+// conversions at load and store; their radix-16 passes run as 16-point DFT-matrix products on MFMA.
 #include "fft_pow2.hpp"
 #include "fft_timing.hpp"
 
