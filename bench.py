@@ -207,6 +207,98 @@ def precision_config(torch, kfr_amd, stream, reps: int):
     return res
 
 
+def _timed(torch, stream, fn, reps):
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def round_trip_config(torch, kfr_amd, stream, reps: int):
+    """Complex-f32 N = 4096, batch 256, re and im i.i.d. U[-1, 1) (seed 1): forward then inverse. The
+    8 MiB batch stays in the 256 MB Infinity Cache, so this is a latency and throughput figure, not an
+    HBM one."""
+    n, batch = 4096, 256
+    g = torch.Generator(device="cuda")
+    g.manual_seed(1)
+    x = torch.rand(batch * n * 2, device="cuda", generator=g) * 2 - 1
+    y = torch.empty_like(x)
+    z = torch.empty_like(x)
+    p = kfr_amd.dft_plan(n, np.complex64)
+
+    def fn():
+        p.execute_batch(y, x, batch, False, stream)
+        p.execute_batch(z, y, batch, True, stream)
+
+    t = _timed(torch, stream, fn, max(reps, 20))
+    err = float((z / n - x).norm() / x.norm())
+    return {"workload": "complex-f32 N=4096 x 256, U[-1,1) (seed 1), forward then inverse",
+            "ms_round_trip": round(t, 4), "us_per_transform": round(t * 1e3 / (2 * batch), 4),
+            "gflops": round(2 * flops("c2c", n, batch) / (t * 1e-3) * 1e-9, 1),
+            "round_trip_rel_rms": float(f"{err:.3e}")}
+
+
+def conv_config(torch, kfr_amd, stream, reps: int):
+    """FFT convolution at full size: x = 128 signals of 2^20 f32 Gaussian samples (seed 5), h = 128
+    Hann-windowed sincs of 4095 taps (cutoff 0.2 fs) plus 1e-3 N(0,1) (seed 6); y = x * h (length
+    1,052,670), then the adjoint (dx, dh) of the same call, both by overlap-save in blocks of 2^14
+    points (the filter spectra are recomputed in every call). Algorithmic bytes:
+    forward 4 (Nx + Nh + Nx+Nh-1), adjoint 4 (Nx+Nh-1 + Nx + Nh + Nx + Nh) per signal. Errors against
+    torch.fft (rocFFT) in float64 on 2 signals."""
+    B, nx, nh = 128, 1 << 20, 4095
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    x = torch.randn(B, nx, device="cuda", generator=g)
+    g.manual_seed(6)
+    m = torch.arange(nh, device="cuda", dtype=torch.float64) - (nh - 1) / 2
+    base = 0.4 * torch.sinc(0.4 * m) * torch.hann_window(nh, periodic=False, device="cuda", dtype=torch.float64)
+    h = (base.float()[None, :] + 1e-3 * torch.randn(B, nh, device="cuda", generator=g)).contiguous()
+    ny = nx + nh - 1
+    gout = torch.randn(B, ny, device="cuda", generator=g)
+    L = kfr_amd.lib()
+    from kfr_amd.autograd import _plan
+    plan = _plan(torch.float32, nx, nh)
+    y = torch.empty(B, ny, device="cuda")
+    dx = torch.empty_like(x)
+    dh = torch.empty_like(h)
+    st = stream.cuda_stream
+
+    def fwd():
+        assert L.kfr_amd_conv_forward(plan, y.data_ptr(), x.data_ptr(), h.data_ptr(), nh, B, 0, st) == 0
+
+    def bwd():
+        assert L.kfr_amd_conv_backward(plan, dx.data_ptr(), dh.data_ptr(), gout.data_ptr(), x.data_ptr(),
+                                       h.data_ptr(), nh, B, st) == 0
+
+    tf = _timed(torch, stream, fwd, reps)
+    tb = _timed(torch, stream, bwd, reps)
+    bf = 4 * (nx + nh + ny) * B
+    bb = 4 * (ny + nx + nh + nx + nh) * B
+    # float64 reference on 2 signals
+    S = 1 << 21
+    xr = x[:2].double().requires_grad_(True)
+    hr = h[:2].double().requires_grad_(True)
+    yr = torch.fft.irfft(torch.fft.rfft(xr, S) * torch.fft.rfft(hr, S), S)[:, :ny]
+    yr.backward(gout[:2].double())
+
+    def rel(a, b):
+        return float(f"{float((a.double() - b).norm() / b.norm()):.3e}")
+
+    return {"workload": "128 x 2^20 f32 signals, 128 x 4095-tap Hann sinc (0.2 fs) + 1e-3 noise, "
+                        "linear convolution and its adjoint (overlap-save, 2^14-point blocks)",
+            "forward_ms": round(tf, 3), "forward_gbs": round(bf / (tf * 1e-3) * 1e-9, 1),
+            "forward_frac": round(bf / (tf * 1e-3) * 1e-9 / HBM_PEAK_GBS, 4),
+            "adjoint_ms": round(tb, 3), "adjoint_gbs": round(bb / (tb * 1e-3) * 1e-9, 1),
+            "adjoint_frac": round(bb / (tb * 1e-3) * 1e-9 / HBM_PEAK_GBS, 4),
+            "rel_rms_vs_fp64": {"y": rel(y[:2], yr.detach()), "dx": rel(dx[:2], xr.grad),
+                                "dh": rel(dh[:2], hr.grad)}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,6 +310,8 @@ def main() -> int:
                     help="comma-separated subset of configurations kind:dtype:n (e.g. c2c:f32:4096), for profiling")
     ap.add_argument("--no-uneven", action="store_true", help="skip the mixed-length batch measurement")
     ap.add_argument("--no-precision", action="store_true", help="skip the half-precision measurement")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the round-trip (N=4096 x 256) and full-size convolution measurements")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one plain 256 MiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -386,6 +480,11 @@ def main() -> int:
     if not args.no_precision and not args.only:
         precision = precision_config(torch, kfr_amd, stream, max(3, args.steps))
 
+    round_trip = conv = None
+    if not args.no_extra and not args.only:
+        round_trip = round_trip_config(torch, kfr_amd, stream, max(3, args.steps))
+        conv = conv_config(torch, kfr_amd, stream, max(3, args.steps))
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(cfgs)
@@ -411,6 +510,8 @@ def main() -> int:
         "rel_rms_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs.items()},
         "uneven_batch": uneven,
         "precision": precision,
+        "round_trip_4096x256": round_trip,
+        "conv": conv,
         "top_kernels": top,
         "single_kernel_c2c_f32_frac": round(lds_bytes / (lds_t * 1e-3) * 1e-9 / HBM_PEAK_GBS, 4) if lds_t else None,
         "long_c2c_f32_frac": round(fs_bytes / (fs_t * 1e-3) * 1e-9 / HBM_PEAK_GBS, 4) if fs_t else None,

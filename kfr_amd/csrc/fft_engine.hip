@@ -642,6 +642,8 @@ struct gpu_conv<T>::impl
 {
     size_t nx = 0, nh = 0, ny = 0, S = 0;
     std::unique_ptr<typename gpu_dft_real<T>::impl> r;
+    // short filters against long signals: overlap-save in one kernel per call
+    std::unique_ptr<os_conv<T>> os;
     dev_mem X, Hs, G, tmp;
     impl(size_t a, size_t b)
     {
@@ -651,7 +653,15 @@ struct gpu_conv<T>::impl
         nh = b;
         ny = a + b - 1;
         S  = std::max<size_t>(2, next_pow2(ny));
-        r.reset(new typename gpu_dft_real<T>::impl(S, 1));
+        const int L = os_conv<T>::block_log(nx, nh);
+        if (L >= 0 && ((size_t)1 << L) < S)
+        {
+            hipStream_t s = library_stream();
+            os.reset(new os_conv<T>(nx, nh, s));
+            hip_check(hipStreamSynchronize(s), "convolution plan initialisation");
+        }
+        else
+            r.reset(new typename gpu_dft_real<T>::impl(S, 1));
     }
     long long cs() const { return (long long)(S / 2 + 1); }
 };
@@ -677,7 +687,13 @@ template <typename T>
 void gpu_conv<T>::forward(void* y, const void* x, const void* h, long long h_dist, long long batch, bool correlate,
                           hipStream_t s)
 {
-    impl& q            = *p;
+    impl& q = *p;
+    if (q.os)
+    {
+        q.os->forward(static_cast<T*>(y), static_cast<const T*>(x), static_cast<const T*>(h), h_dist, batch,
+                      correlate, s);
+        return;
+    }
     const long long cs = q.cs();
     const long long hb = h_dist == 0 ? 1 : batch;
     q.X.ensure((size_t)batch * cs * sizeof(cx<T>));
@@ -706,7 +722,13 @@ template <typename T>
 void gpu_conv<T>::backward(void* dx, void* dh, const void* g, const void* x, const void* h, long long h_dist,
                            long long batch, hipStream_t s)
 {
-    impl& q            = *p;
+    impl& q = *p;
+    if (q.os)
+    {
+        q.os->backward(static_cast<T*>(dx), static_cast<T*>(dh), static_cast<const T*>(g), static_cast<const T*>(x),
+                       static_cast<const T*>(h), h_dist, batch, s);
+        return;
+    }
     const long long cs = q.cs();
     const long long hb = h_dist == 0 ? 1 : batch;
     q.G.ensure((size_t)batch * cs * sizeof(cx<T>));

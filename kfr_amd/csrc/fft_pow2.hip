@@ -749,6 +749,257 @@ inline int log2_exact(size_t n)
     return l;
 }
 
+// ---- FFT convolution by overlap-save, one kernel per call.
+// Output m of the linear convolution y = u * f (|f| = nf) needs u[m - nf + 1 .. m]. A segment of
+// lv = N - nf + 1 consecutive outputs therefore reads N consecutive inputs z[n] = u[i0 + n], and the
+// circular convolution of z with f zero-padded to N equals the linear one at n >= nf - 1. Each
+// workgroup takes two adjacent segments of one signal as the real and imaginary parts of one complex
+// sequence (f is real, so both convolutions come out separated), runs the forward transform, the
+// product with the filter spectrum, and the inverse transform in registers and LDS, and writes
+// the valid outputs: every signal value is read (N / lv) times and every output written once.
+template <typename T>
+struct conv_os_args
+{
+    const T* u       = nullptr; // input signals, item b at b * du
+    T* y             = nullptr; // outputs, item b at b * dy
+    const cx<T>* H   = nullptr; // filter spectra (N values each), item b's at b * hd (hd = 0: shared)
+    const cx<T>* tw  = nullptr; // pass tables of N points
+    long long du     = 0, dy = 0, hd = 0;
+    long long nu     = 0; // input length
+    long long o0     = 0; // full-convolution index of output 0
+    long long nout   = 0; // outputs per item
+    long long pairs  = 0; // segment pairs per item
+    long long count  = 0; // items * pairs
+    int nf           = 1; // filter length
+    int lv           = 1; // valid outputs per segment, N - nf + 1
+    T scale          = T(1); // 1 / N
+};
+
+template <typename T, int L>
+__global__ void __launch_bounds__((shape<T, L>::TPT)) k_conv_os(conv_os_args<T> a)
+{
+    using S          = shape<T, L>;
+    constexpr int E  = S::E;
+    constexpr int TP = S::TPT;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+    const mfma16 K{};
+    const long long w = blockIdx.x;
+    if (w >= a.count)
+        return;
+    const long long item = w / a.pairs;
+    const long long pair = w - item * a.pairs;
+    const int t          = threadIdx.x;
+    const T* u           = a.u + item * a.du;
+    const long long sa   = 2 * pair;
+    const long long ia   = a.o0 + sa * a.lv - (a.nf - 1); // input index of z[0], first segment
+    const long long ib   = ia + a.lv;                      // second segment
+    cx<T> v[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k)
+    {
+        const long long n = t + k * TP;
+        const long long pa = ia + n, pb = ib + n;
+        v[k].re = (pa >= 0 && pa < a.nu) ? u[pa] : T(0);
+        v[k].im = (pb >= 0 && pb < a.nu) ? u[pb] : T(0);
+    }
+    run_pass<T, L, false, role_direct, prec_full, 0>(v, lds, 0, t, a.tw, K);
+    // v[k] = Z[t + k*TPT]
+    const cx<T>* H = a.H + item * a.hd;
+#pragma unroll
+    for (int k = 0; k < E; ++k)
+    {
+        const cx<T> h = H[t + k * TP];
+        const cx<T> p = cmul(v[k], h);
+        v[k]          = { p.re * a.scale, p.im * a.scale };
+    }
+    run_pass<T, L, true, role_direct, prec_full, 0>(v, lds, 0, t, a.tw, K);
+    T* y = a.y + item * a.dy;
+#pragma unroll
+    for (int k = 0; k < E; ++k)
+    {
+        const int n = t + k * TP;
+        if (n >= a.nf - 1)
+        {
+            const long long j  = n - (a.nf - 1);
+            const long long ma = sa * a.lv + j;
+            const long long mb = ma + a.lv;
+            if (ma < a.nout)
+                y[ma] = v[k].re;
+            if (mb < a.nout)
+                y[mb] = v[k].im;
+        }
+    }
+}
+
+// Partial cross-correlations for the filter gradient: segment s of length P = N - nh + 1 of item b
+// gives part[(b*nseg + s)*nh + j] = sum_{i<P} g[sP + i + j] x[sP + i], j < nh. One complex transform
+// of z = x_s + i g_s (x zero beyond P) gives both spectra, X = (Z(k) + conj Z(-k))/2 and
+// G = (Z(k) - conj Z(-k))/(2i); the inverse transform of G conj(X) is the circular correlation, which
+// does not wrap for j < nh because P + nh - 1 = N.
+template <typename T>
+struct xcorr_args
+{
+    const T* x      = nullptr; // item b at b * dx, length nx
+    const T* g      = nullptr; // item b at b * dg, length ng
+    T* part         = nullptr;
+    const cx<T>* tw = nullptr;
+    long long dx = 0, dg = 0, nx = 0, ng = 0;
+    long long nseg  = 0, count = 0;
+    int P = 1, nh = 1;
+    T scale = T(1); // 1 / N
+};
+
+template <typename T, int L>
+__global__ void __launch_bounds__((shape<T, L>::TPT)) k_conv_xcorr(xcorr_args<T> a)
+{
+    using S          = shape<T, L>;
+    constexpr int N  = S::N;
+    constexpr int E  = S::E;
+    constexpr int TP = S::TPT;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+    const mfma16 K{};
+    const long long w = blockIdx.x;
+    if (w >= a.count)
+        return;
+    const long long item = w / a.nseg;
+    const long long seg  = w - item * a.nseg;
+    const int t          = threadIdx.x;
+    const T* x           = a.x + item * a.dx;
+    const T* g           = a.g + item * a.dg;
+    const long long base = seg * a.P;
+    cx<T> v[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k)
+    {
+        const int n       = t + k * TP;
+        const long long p = base + n;
+        v[k].re           = (n < a.P && p < a.nx) ? x[p] : T(0);
+        v[k].im           = p < a.ng ? g[p] : T(0);
+    }
+    run_pass<T, L, false, role_direct, prec_full, 0>(v, lds, 0, t, a.tw, K);
+    // v[k] = Z[t + k*TPT]; fetch Z[-n] through LDS
+#pragma unroll
+    for (int k = 0; k < E; ++k)
+        lds[laddr<role_direct, 1, N>(0, t + k * TP)] = v[k];
+    lds_barrier();
+#pragma unroll
+    for (int k = 0; k < E; ++k)
+    {
+        const int n      = t + k * TP;
+        const cx<T> zm   = lds[laddr<role_direct, 1, N>(0, (N - n) & (N - 1))];
+        const cx<T> z    = v[k];
+        const cx<T> X    = { T(0.5) * (z.re + zm.re), T(0.5) * (z.im - zm.im) };
+        // G = (z - conj zm) / (2i)
+        const cx<T> dgi  = { z.re - zm.re, z.im + zm.im };
+        const cx<T> G    = { T(0.5) * dgi.im, -T(0.5) * dgi.re };
+        // G conj(X) / N
+        v[k] = { (G.re * X.re + G.im * X.im) * a.scale, (G.im * X.re - G.re * X.im) * a.scale };
+    }
+    lds_barrier();
+    run_pass<T, L, true, role_direct, prec_full, 0>(v, lds, 0, t, a.tw, K);
+    T* out = a.part + w * a.nh;
+#pragma unroll
+    for (int k = 0; k < E; ++k)
+    {
+        const int n = t + k * TP;
+        if (n < a.nh)
+            out[n] = v[k].re;
+    }
+}
+
+// dh[b][j] = sum over the segments (and over the items when the filter is shared) of the partials
+template <typename T>
+__global__ void k_conv_xcorr_sum(T* dh, const T* part, long long nh, long long nseg, long long items,
+                                 bool shared)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long outs = shared ? nh : items * nh;
+    if (i >= outs)
+        return;
+    const long long b0 = shared ? 0 : i / nh, b1 = shared ? items : b0 + 1;
+    const long long j  = shared ? i : i - b0 * nh;
+    T acc              = T(0);
+    for (long long b = b0; b < b1; ++b)
+        for (long long s = 0; s < nseg; ++s)
+            acc += part[(b * nseg + s) * nh + j];
+    dh[i] = acc;
+}
+
+// complex filter buffer (f[n], 0) for n < nf, zero to N; f = h reversed when rev
+template <typename T>
+__global__ void k_conv_filter_pad(cx<T>* out, const T* h, long long hd, int nf, int N, long long count, bool rev)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count * N)
+        return;
+    const long long b = i / N;
+    const int n       = (int)(i - b * N);
+    const T* hb       = h + b * hd;
+    out[i]            = { n < nf ? hb[rev ? nf - 1 - n : n] : T(0), T(0) };
+}
+
+template <typename T, int L>
+void conv_os_launch(const conv_os_args<T>& a, hipStream_t s)
+{
+    using S                  = shape<T, L>;
+    constexpr size_t bytes   = (size_t)(S::N + S::N / 16 + 1) * sizeof(cx<T>);
+    static const bool lds_ok = (allow_lds(k_conv_os<T, L>, bytes), true);
+    (void)lds_ok;
+    if (a.count > 0x7fffffffll)
+        throw std::runtime_error("convolution too large for one launch");
+    timing::span ts([] { return timing::kernel_name("k_conv_os", timing::type_name<T>(), L); },
+                    (long long)sizeof(T) * (a.count / a.pairs) * (a.nu + a.nout), s);
+    hipLaunchKernelGGL((k_conv_os<T, L>), dim3((unsigned)a.count), dim3(S::TPT), bytes, s, a);
+    hip_check(hipGetLastError(), "overlap-save convolution launch");
+}
+
+template <typename T, int L>
+void conv_xcorr_launch(const xcorr_args<T>& a, hipStream_t s)
+{
+    using S                  = shape<T, L>;
+    constexpr size_t bytes   = (size_t)(S::N + S::N / 16 + 1) * sizeof(cx<T>);
+    static const bool lds_ok = (allow_lds(k_conv_xcorr<T, L>, bytes), true);
+    (void)lds_ok;
+    if (a.count > 0x7fffffffll)
+        throw std::runtime_error("correlation too large for one launch");
+    timing::span ts([] { return timing::kernel_name("k_conv_xcorr", timing::type_name<T>(), L); },
+                    (long long)sizeof(T) * (a.count / a.nseg) * (a.nx + a.ng) +
+                        (long long)sizeof(T) * a.count * a.nh,
+                    s);
+    hipLaunchKernelGGL((k_conv_xcorr<T, L>), dim3((unsigned)a.count), dim3(S::TPT), bytes, s, a);
+    hip_check(hipGetLastError(), "correlation launch");
+}
+
+constexpr int conv_min_log = 8;
+
+template <typename T, int LO>
+void conv_os_any(int L, const conv_os_args<T>& a, hipStream_t s)
+{
+    if constexpr (LO <= max_log<T>)
+    {
+        if (L == LO)
+            return conv_os_launch<T, LO>(a, s);
+        return conv_os_any<T, LO + 1>(L, a, s);
+    }
+    else
+        throw std::logic_error("no overlap-save kernel for this block length");
+}
+
+template <typename T, int LO>
+void conv_xcorr_any(int L, const xcorr_args<T>& a, hipStream_t s)
+{
+    if constexpr (LO <= max_log<T>)
+    {
+        if (L == LO)
+            return conv_xcorr_launch<T, LO>(a, s);
+        return conv_xcorr_any<T, LO + 1>(L, a, s);
+    }
+    else
+        throw std::logic_error("no correlation kernel for this block length");
+}
+
 } // namespace p2
 
 template <typename T>
@@ -1005,5 +1256,152 @@ std::string pow2_fft<T>::describe() const
 
 template class pow2_fft<float>;
 template class pow2_fft<double>;
+
+// ------------------------------------------------------------------------------------------------
+
+template <typename T>
+struct os_conv<T>::impl
+{
+    size_t nx = 0, nh = 0;
+    int L     = 0;
+    dev_mem tw, hz, H, part;
+    std::unique_ptr<pow2_fft<T>> fft; // filter spectra
+};
+
+template <typename T>
+int os_conv<T>::block_log(size_t nx, size_t nh)
+{
+    (void)nx;
+    if (nh == 0)
+        return -1;
+    // the shortest block whose valid part is at least two thirds of it (half at the longest block)
+    for (int L = p2::conv_min_log; L <= p2::max_log<T>; ++L)
+        if (3 * (nh - 1) <= ((size_t)1 << L))
+            return L;
+    return 2 * (nh - 1) <= ((size_t)1 << p2::max_log<T>) ? p2::max_log<T> : -1;
+}
+
+template <typename T>
+os_conv<T>::os_conv(size_t nx, size_t nh, hipStream_t s) : p(new impl)
+{
+    const int L = block_log(nx, nh);
+    if (L < 0)
+        throw std::invalid_argument("os_conv: filter too long for an LDS block");
+    p->nx = nx;
+    p->nh = nh;
+    p->L  = L;
+    const int le  = p2::log_e<T>;
+    const int cnt = p2::pass_twoff(L, le, p2::pass_count(L, le));
+    p->tw.reset((size_t)std::max(1, cnt) * sizeof(cx<T>));
+    p2::gen_pass_tables<T>(L, p->tw.template as<cx<T>>(), s);
+    p->fft.reset(new pow2_fft<T>((size_t)1 << L, s));
+}
+
+template <typename T>
+os_conv<T>::~os_conv() = default;
+
+template <typename T>
+std::string os_conv<T>::describe() const
+{
+    return "overlap-save block=" + std::to_string(1 << p->L) + " valid=" +
+           std::to_string((1 << p->L) - (long long)p->nh + 1);
+}
+
+// spectra of the filters (reversed when rev) scaled later by 1/N inside the kernels
+template <typename T>
+static const cx<T>* filter_spectra(typename os_conv<T>::impl& q, const T* h, long long h_dist, long long nfilt,
+                                   bool rev, hipStream_t s)
+{
+    const long long N = 1ll << q.L;
+    q.hz.ensure((size_t)(nfilt * N) * sizeof(cx<T>));
+    q.H.ensure((size_t)(nfilt * N) * sizeof(cx<T>));
+    hipLaunchKernelGGL(p2::k_conv_filter_pad<T>, dim3(nblocks(nfilt * N, 256)), dim3(256), 0, s,
+                       q.hz.template as<cx<T>>(), h, h_dist, (int)q.nh, (int)N, nfilt, rev);
+    hip_check(hipGetLastError(), "filter padding");
+    q.fft->run(false, q.hz.ptr, q.H.ptr, nfilt, s);
+    return q.H.template as<cx<T>>();
+}
+
+template <typename T>
+static void run_os(typename os_conv<T>::impl& q, T* y, const T* u, long long nu, long long du, long long o0,
+                   long long nout, long long dy, const cx<T>* H, long long hd, long long batch, hipStream_t s)
+{
+    const int N = 1 << q.L;
+    p2::conv_os_args<T> a;
+    a.u     = u;
+    a.y     = y;
+    a.H     = H;
+    a.tw    = q.tw.template as<cx<T>>();
+    a.du    = du;
+    a.dy    = dy;
+    a.hd    = hd;
+    a.nu    = nu;
+    a.o0    = o0;
+    a.nout  = nout;
+    a.nf    = (int)q.nh;
+    a.lv    = N - (int)q.nh + 1;
+    const long long nseg = (nout + a.lv - 1) / a.lv;
+    a.pairs = (nseg + 1) / 2;
+    a.count = a.pairs * batch;
+    a.scale = T(1) / T(N);
+    if (a.count > 0)
+        p2::conv_os_any<T, p2::conv_min_log>(q.L, a, s);
+}
+
+template <typename T>
+void os_conv<T>::forward(T* y, const T* x, const T* h, long long h_dist, long long batch, bool correlate,
+                         hipStream_t s)
+{
+    impl& q              = *p;
+    const long long nfil = h_dist == 0 ? 1 : batch;
+    const cx<T>* H       = filter_spectra<T>(q, h, h_dist, nfil, correlate, s);
+    const long long ny   = (long long)(q.nx + q.nh - 1);
+    run_os<T>(q, y, x, (long long)q.nx, (long long)q.nx, 0, ny, ny, H, h_dist == 0 ? 0 : (1ll << q.L), batch, s);
+}
+
+template <typename T>
+void os_conv<T>::backward(T* dx, T* dh, const T* g, const T* x, const T* h, long long h_dist, long long batch,
+                          hipStream_t s)
+{
+    impl& q            = *p;
+    const long long ny = (long long)(q.nx + q.nh - 1);
+    const long long N  = 1ll << q.L;
+    if (dx)
+    {
+        // dx[n] = sum_j g[n + j] h[j] = (g * reversed h)[n + nh - 1]
+        const long long nfil = h_dist == 0 ? 1 : batch;
+        const cx<T>* H       = filter_spectra<T>(q, h, h_dist, nfil, true, s);
+        run_os<T>(q, dx, g, ny, ny, (long long)q.nh - 1, (long long)q.nx, (long long)q.nx, H,
+                  h_dist == 0 ? 0 : N, batch, s);
+    }
+    if (dh)
+    {
+        p2::xcorr_args<T> a;
+        a.x     = x;
+        a.g     = g;
+        a.tw    = q.tw.template as<cx<T>>();
+        a.dx    = (long long)q.nx;
+        a.dg    = ny;
+        a.nx    = (long long)q.nx;
+        a.ng    = ny;
+        a.P     = (int)(N - (long long)q.nh + 1);
+        a.nh    = (int)q.nh;
+        a.nseg  = ((long long)q.nx + a.P - 1) / a.P;
+        a.count = a.nseg * batch;
+        a.scale = T(1) / T(N);
+        q.part.ensure((size_t)(a.count * a.nh) * sizeof(T));
+        a.part = q.part.template as<T>();
+        if (a.count > 0)
+            p2::conv_xcorr_any<T, p2::conv_min_log>(q.L, a, s);
+        const bool shared    = h_dist == 0;
+        const long long outs = shared ? (long long)q.nh : batch * (long long)q.nh;
+        hipLaunchKernelGGL(p2::k_conv_xcorr_sum<T>, dim3(nblocks(outs, 256)), dim3(256), 0, s, dh, a.part,
+                           (long long)q.nh, a.nseg, batch, shared);
+        hip_check(hipGetLastError(), "gradient reduction");
+    }
+}
+
+template class os_conv<float>;
+template class os_conv<double>;
 
 } // namespace kfr_amd

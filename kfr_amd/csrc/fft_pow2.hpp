@@ -35,4 +35,33 @@ private:
 extern template class pow2_fft<float>;
 extern template class pow2_fft<double>;
 
+// Batched linear convolution of real signals with a short filter by overlap-save: one kernel per call
+// transforms blocks of 2^L points (two segments per complex transform), multiplies by the filter
+// spectrum and transforms back in LDS. Same contract as gpu_conv (convolve / correlate and the
+// adjoint); available when the filter fits an LDS block with room for the valid outputs.
+template <typename T>
+class os_conv
+{
+public:
+    // log2 of the block length for these lengths, or -1 when the filter is too long
+    static int block_log(size_t nx, size_t nh);
+    os_conv(size_t nx, size_t nh, hipStream_t s);
+    ~os_conv();
+    // y[b] = x[b] * h[b] (h_dist = 0: one shared filter); correlate: h reversed. |y| = nx + nh - 1
+    void forward(T* y, const T* x, const T* h, long long h_dist, long long batch, bool correlate, hipStream_t s);
+    // dx = g correlated with h (first nx values), dh = g correlated with x (first nh lags; summed over
+    // the batch when h_dist = 0); either may be null
+    void backward(T* dx, T* dh, const T* g, const T* x, const T* h, long long h_dist, long long batch,
+                  hipStream_t s);
+    std::string describe() const;
+
+    struct impl;
+
+private:
+    std::unique_ptr<impl> p;
+};
+
+extern template class os_conv<float>;
+extern template class os_conv<double>;
+
 } // namespace kfr_amd

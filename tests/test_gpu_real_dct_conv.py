@@ -213,3 +213,52 @@ def test_batched_conv_adjoint_matches_torch_fft_reference():
     assert oracle.rel_rms(y.detach().cpu().numpy(), yr.detach().numpy()) < 1e-5
     assert oracle.rel_rms(x.grad.cpu().numpy(), xr.grad.numpy()) < 1e-5
     assert oracle.rel_rms(h.grad.cpu().numpy(), hr.grad.numpy()) < 1e-5
+
+
+def test_conv_full_size_per_signal_taps():
+    """the full-size convolution workload: 128 signals of 2^20 samples, each with its own 4095-tap
+    Hann-windowed sinc (cutoff 0.2 fs, plus 1e-3 noise); output and adjoint on three sampled signals
+    against torch.fft (rocFFT) in float64"""
+    torch = pytest.importorskip("torch")
+    from kfr_amd.autograd import fft_convolve
+    B, nx, nh = 128, 1 << 20, 4095
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    x = torch.randn(B, nx, device="cuda", generator=g).requires_grad_(True)
+    m = torch.arange(nh, device="cuda", dtype=torch.float64) - (nh - 1) / 2
+    base = 0.4 * torch.sinc(0.4 * m) * torch.hann_window(nh, periodic=False, device="cuda", dtype=torch.float64)
+    h = (base.float()[None, :] + 1e-3 * torch.randn(B, nh, device="cuda", generator=g)).requires_grad_(True)
+    y = fft_convolve(x, h)
+    gout = torch.randn(B, nx + nh - 1, device="cuda", generator=g)
+    y.backward(gout)
+    torch.cuda.synchronize()
+    pick = [0, 77, 127]
+    S = 1 << 21
+    xr = x.detach()[pick].double().requires_grad_(True)
+    hr = h.detach()[pick].double().requires_grad_(True)
+    yr = torch.fft.irfft(torch.fft.rfft(xr, S) * torch.fft.rfft(hr, S), S)[:, : nx + nh - 1]
+    yr.backward(gout[pick].double())
+
+    def rel(a, b):
+        return float((a.detach().double() - b.detach()).norm() / b.detach().norm())
+
+    ey, ex, eh = rel(y[pick], yr), rel(x.grad[pick], xr.grad), rel(h.grad[pick], hr.grad)
+    print(f"full-size conv rel_rms y={ey:.3e} dx={ex:.3e} dh={eh:.3e}", flush=True)
+    assert ey < 1e-6 and ex < 1e-6 and eh < 1e-6
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("nx,nh", [(100000, 3), (50001, 257), (70000, 1500), (1 << 15, 2049), (200003, 4001)])
+def test_overlap_save_lengths(dt, nx, nh):
+    """long signals with short filters take the overlap-save kernels: odd segment counts, a last
+    segment pair with one segment, filters up to half the block; convolve and correlate against
+    numpy in FP64"""
+    a = oracle.gaussian(31 + nh, nx, dt)
+    b = oracle.gaussian(37 + nh, nh, dt)
+    tol = 1e-5 if dt == np.float32 else 1e-12
+    ref = np.convolve(a.astype(np.float64), b.astype(np.float64))
+    e = oracle.rel_rms(kfr_amd.convolve(a, b), ref)
+    refc = np.convolve(a.astype(np.float64), b[::-1].astype(np.float64))
+    ec = oracle.rel_rms(kfr_amd.correlate(a, b), refc)
+    print(f"overlap-save {np.dtype(dt).name} nx={nx} nh={nh} conv={e:.3e} corr={ec:.3e}", flush=True)
+    assert e < tol and ec < tol
