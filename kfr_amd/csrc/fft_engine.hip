@@ -539,19 +539,164 @@ void gpu_dft_real<T>::inverse(void* rout, const void* cin, long long batch, hipS
 // even/odd mirrored input, y_k = Re(Y_k W_{4n}^k); the inverse (DCT-III) pre-multiplies by
 // W_{4n}^k with x_0 halved and de-interleaves the real parts.
 
+// Lengths n = 4m whose half length has a plain fast path use a real transform of the permuted input
+// instead (Makhoul): v = x mirrored (v[e] = x[2e], v[n-1-e] = x[2e+1]) is packed as n/2 complex values
+// z[q] = (v[2q], v[2q+1]) by one coalesced pass, transformed with the n/2-point plan, and one pass
+// forms the real spectrum V[k] = A[k] + W_n^k B[k] (A, B the spectra of the even and odd samples,
+// from Z[k] and Z[n/2-k]) and writes y[k] = Re(V[k] W_{4n}^k) and y[n-k] = -Im(V[k] W_{4n}^k). DCT-III
+// runs the same steps backwards: the spectrum of the symmetrised pre-twiddled input, an n/2-point
+// inverse transform, and the permutation back.
+
+// z[q] = (x[4q], x[4q+2]), z[n/2-1-q] = (x[4q+3], x[4q+1]) for q < n/4: the mirror permutation
+template <typename T>
+__global__ void k_dct2_pack(cx<T>* z, const T* x, long long n, long long count)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long quarter = n / 4;
+    if (i >= count * quarter)
+        return;
+    const long long b = i / quarter;
+    const long long q = i - b * quarter;
+    const T* xb       = x + b * n;
+    cx<T>* zb         = z + b * (n / 2);
+    const T x0 = xb[4 * q], x1 = xb[4 * q + 1], x2 = xb[4 * q + 2], x3 = xb[4 * q + 3];
+    zb[q]              = { x0, x2 };
+    zb[n / 2 - 1 - q]  = { x3, x1 };
+}
+
+// inverse of k_dct2_pack: x[4q..4q+3] = (u[q].re, u[h-1-q].im, u[q].im, u[h-1-q].re)
+template <typename T>
+__global__ void k_dct3_unpack(T* x, const cx<T>* u, long long n, long long count)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long quarter = n / 4;
+    if (i >= count * quarter)
+        return;
+    const long long b = i / quarter;
+    const long long q = i - b * quarter;
+    T* xb             = x + b * n;
+    const cx<T>* ub   = u + b * (n / 2);
+    const cx<T> a = ub[q], c = ub[n / 2 - 1 - q];
+    xb[4 * q]     = a.re;
+    xb[4 * q + 1] = c.im;
+    xb[4 * q + 2] = a.im;
+    xb[4 * q + 3] = c.re;
+}
+
+// DCT-II output from Z = FFT_{n/2}(z): thread k (k <= n/4) handles the pair k, n/2 - k
+template <typename T>
+__global__ void k_dct2_post(T* y, const cx<T>* Z, const cx<T>* wn, const cx<T>* w4n, long long n, long long count)
+{
+    const long long i  = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long h  = n / 2;
+    const long long pr = n / 4 + 1;
+    if (i >= count * pr)
+        return;
+    const long long b = i / pr;
+    const long long k = i - b * pr;
+    const cx<T>* Zb   = Z + b * h;
+    T* yb             = y + b * n;
+    const cx<T> zk = Zb[k == 0 ? 0 : k], zm = Zb[k == 0 ? 0 : h - k];
+    // V[k] = (Zk + conj Zm)/2 + W_n^k (Zk - conj Zm)/(2i); V[h-k] likewise with the roles swapped
+    auto spec = [&](cx<T> a, cx<T> c, long long kk) -> cx<T>
+    {
+        const cx<T> A  = { T(0.5) * (a.re + c.re), T(0.5) * (a.im - c.im) };
+        const cx<T> d  = { a.re - c.re, a.im + c.im }; // a - conj c
+        const cx<T> B  = { T(0.5) * d.im, -T(0.5) * d.re }; // d / (2i)
+        const cx<T> w  = wn[kk];
+        return { A.re + w.re * B.re - w.im * B.im, A.im + w.re * B.im + w.im * B.re };
+    };
+    auto emit = [&](cx<T> V, long long kk)
+    {
+        const cx<T> w = w4n[kk];
+        const T re    = V.re * w.re - V.im * w.im;
+        const T im    = V.re * w.im + V.im * w.re;
+        yb[kk] = re;
+        if (kk > 0 && n - kk != kk)
+            yb[n - kk] = -im;
+    };
+    if (k == 0)
+    {
+        // V[0] = Re Z0 + Im Z0 (both real), V[h] = Re Z0 - Im Z0
+        emit({ zk.re + zk.im, T(0) }, 0);
+        const cx<T> Vh = { zk.re - zk.im, T(0) };
+        emit(Vh, h);
+        return;
+    }
+    emit(spec(zk, zm, k), k);
+    if (h - k != k)
+        emit(spec(zm, zk, h - k), h - k);
+}
+
+// DCT-III input side: V[k] = y'[k] W_{4n}^k (y'[0] = y[0]/2); the real part of DFT_n(V) is the
+// unscaled inverse real transform of H[k] = (conj V[k] + V[n-k])/2, whose n/2-point packing is
+// Z'[k] = (H[k] + conj H[h-k]) + i (H[k] - conj H[h-k]) conj(W_n^k)
+template <typename T>
+__global__ void k_dct3_pre(cx<T>* Zp, const T* y, const cx<T>* wn, const cx<T>* w4n, long long n, long long count)
+{
+    const long long i  = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long h  = n / 2;
+    const long long pr = n / 4 + 1;
+    if (i >= count * pr)
+        return;
+    const long long b = i / pr;
+    const long long k = i - b * pr;
+    const T* yb       = y + b * n;
+    cx<T>* Zb         = Zp + b * h;
+    auto Hs = [&](long long kk) -> cx<T>
+    {
+        // H[kk] = (conj V[kk] + V[n-kk]) / 2, V[n] = V[0]
+        const T a0        = yb[kk] * (kk == 0 ? T(0.5) : T(1));
+        const cx<T> w0    = w4n[kk];
+        const cx<T> V0    = { a0 * w0.re, a0 * w0.im };
+        const long long m = kk == 0 ? 0 : n - kk;
+        const T a1        = yb[m] * (m == 0 ? T(0.5) : T(1));
+        const cx<T> w1    = w4n[m];
+        const cx<T> V1    = { a1 * w1.re, a1 * w1.im };
+        return { T(0.5) * (V0.re + V1.re), T(0.5) * (V1.im - V0.im) };
+    };
+    auto pack = [&](cx<T> a, cx<T> c, long long kk) -> cx<T>
+    {
+        // (a + conj c) + i (a - conj c) conj(W_n^kk)
+        const cx<T> s = { a.re + c.re, a.im - c.im };
+        const cx<T> d = { a.re - c.re, a.im + c.im };
+        const cx<T> w = wn[kk];
+        const cx<T> e = { d.re * w.re + d.im * w.im, d.im * w.re - d.re * w.im }; // d conj(w)
+        return { s.re - e.im, s.im + e.re };
+    };
+    const cx<T> Hk = Hs(k);
+    const cx<T> Hm = Hs(h - k);
+    Zb[k == h ? 0 : k] = pack(Hk, Hm, k);
+    if (k != 0 && h - k != k)
+        Zb[h - k] = pack(Hm, Hk, h - k);
+}
+
 template <typename T>
 struct gpu_dct<T>::impl
 {
     size_t n = 0;
-    std::unique_ptr<typename gpu_dft<T>::impl> c;
-    dev_mem w; // W_{4n}^k, k < n
+    std::unique_ptr<typename gpu_dft<T>::impl> c;  // n-point complex plan (generic path)
+    std::unique_ptr<typename gpu_dft<T>::impl> ch; // n/2-point complex plan (real-transform path)
+    dev_mem w;  // W_{4n}^k, k < n
+    dev_mem wn; // W_n^k, k <= n/2
+    dev_mem z;  // packed n/2-point data of one round
+    long long chunk = 1;
     explicit impl(size_t len)
     {
-        n = len;
-        c.reset(new typename gpu_dft<T>::impl(n));
+        n             = len;
         hipStream_t s = library_stream();
         w.reset(std::max<size_t>(1, n) * sizeof(cx<T>));
         gen_twiddles(w.as<cx<T>>(), (long long)n, 4 * (long long)n, 1, 0, tw_linear, 0, s);
+        const size_t h = n / 2;
+        if (n % 4 == 0 && n >= 16 && (pow2_fft<T>::supports(h) || mixed_fft<T>::supports(h)))
+        {
+            ch.reset(new typename gpu_dft<T>::impl(h));
+            wn.reset((h + 1) * sizeof(cx<T>));
+            gen_twiddles(wn.as<cx<T>>(), (long long)h + 1, (long long)n, 1, 0, tw_linear, 0, s);
+            chunk = (long long)std::max<size_t>(1, ((size_t)512 << 20) / (h * sizeof(cx<T>)));
+        }
+        else
+            c.reset(new typename gpu_dft<T>::impl(n));
         hip_check(hipStreamSynchronize(s), "dct plan initialisation");
     }
 };
@@ -570,12 +715,63 @@ size_t gpu_dct<T>::size() const
 template <typename T>
 const schedule_info& gpu_dct<T>::schedule() const
 {
-    return p->c->info;
+    return p->ch ? p->ch->info : p->c->info;
 }
 template <typename T>
 void gpu_dct<T>::execute(void* out, const void* in, long long batch, bool inverse, hipStream_t s)
 {
     const long long n = (long long)p->n;
+    if (p->ch)
+    {
+        impl& q           = *p;
+        const long long h = n / 2;
+        q.z.ensure((size_t)(std::min(batch, q.chunk) * h) * sizeof(cx<T>));
+        cx<T>* z = q.z.template as<cx<T>>();
+        for (long long b0 = 0; b0 < batch; b0 += q.chunk)
+        {
+            const long long nb = std::min(q.chunk, batch - b0);
+            const T* x         = static_cast<const T*>(in) + b0 * n;
+            T* y               = static_cast<T*>(out) + b0 * n;
+            const long long pr = n / 4 + 1;
+            if (!inverse)
+            {
+                {
+                    timing::span ts([] { return timing::kernel_name("k_dct2_pack", timing::type_name<T>()); },
+                                    2 * (long long)sizeof(T) * nb * n, s);
+                    hipLaunchKernelGGL(k_dct2_pack<T>, dim3(nblocks(nb * (n / 4), 256)), dim3(256), 0, s, z, x, n,
+                                       nb);
+                }
+                hip_check(hipGetLastError(), "dct pack");
+                q.ch->run(load_plain<T>(z, h), store_plain<T>(z, h), nb, false, s);
+                {
+                    timing::span ts([] { return timing::kernel_name("k_dct2_post", timing::type_name<T>()); },
+                                    2 * (long long)sizeof(T) * nb * n, s);
+                    hipLaunchKernelGGL(k_dct2_post<T>, dim3(nblocks(nb * pr, 256)), dim3(256), 0, s, y, z,
+                                       q.wn.template as<cx<T>>(), q.w.template as<cx<T>>(), n, nb);
+                }
+                hip_check(hipGetLastError(), "dct post");
+            }
+            else
+            {
+                {
+                    timing::span ts([] { return timing::kernel_name("k_dct3_pre", timing::type_name<T>()); },
+                                    2 * (long long)sizeof(T) * nb * n, s);
+                    hipLaunchKernelGGL(k_dct3_pre<T>, dim3(nblocks(nb * pr, 256)), dim3(256), 0, s, z, x,
+                                       q.wn.template as<cx<T>>(), q.w.template as<cx<T>>(), n, nb);
+                }
+                hip_check(hipGetLastError(), "dct-iii pre");
+                q.ch->run(load_plain<T>(z, h), store_plain<T>(z, h), nb, true, s);
+                {
+                    timing::span ts([] { return timing::kernel_name("k_dct3_unpack", timing::type_name<T>()); },
+                                    2 * (long long)sizeof(T) * nb * n, s);
+                    hipLaunchKernelGGL(k_dct3_unpack<T>, dim3(nblocks(nb * (n / 4), 256)), dim3(256), 0, s, y, z,
+                                       n, nb);
+                }
+                hip_check(hipGetLastError(), "dct-iii unpack");
+            }
+        }
+        return;
+    }
     xload<T> l;
     xstore<T> st;
     l.p  = in;

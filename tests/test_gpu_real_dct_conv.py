@@ -62,9 +62,11 @@ def test_dct_known_answers():
     assert np.sqrt(np.mean((y - KAT["dct_inv"]) ** 2)) < 1e-5
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 16, 17, 64, 1000, 4096, 65536])
+@pytest.mark.parametrize("n", [1, 2, 3, 8, 12, 16, 17, 64, 1000, 4096, 7680, 8000, 65536, 1 << 20])
 @pytest.mark.parametrize("dt", [np.float32, np.float64])
 def test_dct(n, dt):
+    """lengths 4m whose half has a power-of-two or menu plan take the half-length real-transform path
+    (8, 16, 64, 4096, 7680, 8000, 65536, 2^20), the others the n-point generic path"""
     x = oracle.gaussian(900 + n, n, dt)
     p = kfr_amd.dct_plan(n, dt)
     for inv in (False, True):
@@ -87,6 +89,11 @@ def test_dct_batch_65536_x_1024_sample():
     y = yd.cpu().numpy().reshape(b, n)
     ref = fp64_dct(x.reshape(b, n)[:4].astype(np.float64))
     assert max(oracle.rel_rms(y[i], ref[i]) for i in range(4)) < 2e-6
+    p.execute_batch(yd, xd, b, True)
+    torch.cuda.synchronize()
+    y = yd.cpu().numpy().reshape(b, n)
+    ref = fp64_dct(x.reshape(b, n)[-4:].astype(np.float64), True)
+    assert max(oracle.rel_rms(y[b - 4 + i], ref[i]) for i in range(4)) < 2e-6
 
 
 def test_convolution_known_answers():
