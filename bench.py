@@ -449,7 +449,7 @@ def main() -> int:
     # achieved = its algorithmic bytes per launch / its average launch duration
     kernels = kt.by_kernel()
     top = []
-    for name, k in list(kernels.items())[:8]:
+    for name, k in list(kernels.items())[: (None if args.only else 8)]:
         avg_ms = k["ms"] / k["launches"]
         gbs = k["bytes"] / (k["ms"] * 1e-3) * 1e-9 if k["ms"] > 0 else 0.0
         top.append({"kernel": name, "launches_per_step": k["launches"] / args.steps, "avg_us": round(avg_ms * 1e3, 1),
