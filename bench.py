@@ -23,6 +23,7 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
+ULP_EDGES = (0.5, 1.0, 2.0, 4.0, 8.0, 16.0, 64.0)  # ulp histogram bucket edges
 # HBM bytes per dispatch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this benchmark
 TRAFFIC_TABLE = "bench_sweep_hbm_traffic.md"
 
@@ -311,7 +312,7 @@ def main() -> int:
     ap.add_argument("--no-uneven", action="store_true", help="skip the mixed-length batch measurement")
     ap.add_argument("--no-precision", action="store_true", help="skip the half-precision measurement")
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the round-trip (N=4096 x 256) and full-size convolution measurements")
+                    help="skip the round-trip (N=4096 x 256), full-size convolution and device-copy measurements")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one plain 256 MiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -441,9 +442,38 @@ def main() -> int:
             x = src32[: n * nsamp].cpu().numpy().reshape(nsamp, n)
             y = dst[: n * nsamp].cpu().numpy().reshape(nsamp, n)
             ref = scipy.fft.dct(x.astype(np.float64), type=2, axis=-1) / 2.0
-        e = max(float(np.linalg.norm(y[b] - ref[b]) / np.linalg.norm(ref[b])) for b in range(nsamp))
+        rel = [float(np.linalg.norm(y[b] - ref[b]) / np.linalg.norm(ref[b])) for b in range(nsamp)]
         k = f"{kind}_{dt}"
-        errs[k] = max(errs.get(k, 0.0), e)
+        a = errs.setdefault(k, {"max": 0.0, "sum": 0.0, "count": 0, "max_abs_rel": 0.0})
+        a["max"] = max(a["max"], max(rel))
+        a["sum"] += sum(rel)
+        a["count"] += len(rel)
+        a["max_abs_rel"] = max(a["max_abs_rel"], float(np.max(np.abs(y - ref)) / np.max(np.abs(ref))))
+        if k == "c2c_f32":
+            # error of each real component in units in the last place of its FP64 value rounded to f32
+            yr = np.stack([y.real, y.imag]).ravel().astype(np.float64)
+            rr = np.stack([ref.real, ref.imag]).ravel()
+            ulp = np.spacing(np.abs(rr).astype(np.float32)).astype(np.float64)
+            u = np.abs(yr - rr) / ulp
+            hist = errs.setdefault("ulp_hist", np.zeros(len(ULP_EDGES) + 1, dtype=np.int64))
+            hist += np.bincount(np.searchsorted(ULP_EDGES, u, side="right"), minlength=len(ULP_EDGES) + 1)
+    ulp_hist = errs.pop("ulp_hist", None)
+    err_report = {k: {"rel_rms_max": float(f"{v['max']:.3e}"), "rel_rms_mean": float(f"{v['sum'] / v['count']:.3e}"),
+                      "max_abs_rel": float(f"{v['max_abs_rel']:.3e}"), "transforms": v["count"]}
+                  for k, v in errs.items()}
+    if ulp_hist is not None:
+        labels = ([f"<{ULP_EDGES[0]:g}"] + [f"{ULP_EDGES[i]:g}-{ULP_EDGES[i + 1]:g}" for i in range(len(ULP_EDGES) - 1)]
+                  + [f">={ULP_EDGES[-1]:g}"])
+        err_report["ulp_histogram_c2c_f32"] = {lab: int(c) for lab, c in zip(labels, ulp_hist)}
+
+    # a plain device copy of 1 GiB: the bandwidth a streaming kernel reaches on this device (not in
+    # profiling runs, whose calibration probe is the largest copy)
+    copy_gbs = None
+    if not args.no_extra:
+        copy_dst = torch.empty_like(src32)
+        copy_ms = _timed(torch, stream, lambda: copy_dst.copy_(src32), 5)
+        copy_gbs = 2 * src32.numel() * 4 / (copy_ms * 1e-3) * 1e-9
+        del copy_dst
 
     # dominant kernel: the kernel (template instance) with the largest total time in the timed region;
     # achieved = its algorithmic bytes per launch / its average launch duration
@@ -464,7 +494,9 @@ def main() -> int:
                     "traffic": int(tr) if tr is not None else None,
                     "kernel": d["kernel"], "avg_us": d["avg_us"], "alg_bytes_per_launch": d["alg_bytes_per_launch"],
                     "launches_per_step": d["launches_per_step"], "share_of_step": d["share_of_step"],
-                    "traffic_source": f"profiles/{TRAFFIC_TABLE}" if tr is not None else None}
+                    "traffic_source": f"profiles/{TRAFFIC_TABLE}" if tr is not None else None,
+                    "copy_gbs": round(copy_gbs, 1) if copy_gbs else None,
+                    "frac_of_copy": round(d["gbs"] / copy_gbs, 4) if copy_gbs else None}
     fs = [p for p in per if p["kind"] == "c2c" and p["dtype"] == "f32" and p["n"] > (1 << 14)]
     fs_t = sum(p["ms"] for p in fs)
     fs_bytes = sum(alg_bytes("c2c", "f32", p["n"], p["batch"]) for p in fs)
@@ -507,7 +539,7 @@ def main() -> int:
                    "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "rel_rms_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs.items()},
+        "errors_vs_fp64": err_report,
         "uneven_batch": uneven,
         "precision": precision,
         "round_trip_4096x256": round_trip,
