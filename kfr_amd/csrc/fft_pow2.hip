@@ -100,6 +100,10 @@ struct geom
         int w = 16;
         while (w > 1 && (w * S::N > lds_budget<T> || w * S::TPT > 1024))
             w /= 2;
+        // runs longer than a 128-byte line buy nothing, two workgroups per CU do (measured for f64 at
+        // 2^9 points: 512 threads and 128-byte runs beat 1024 threads and 256-byte runs)
+        if (ROLE != role_pass && w * (int)sizeof(cx<T>) > 128 && w * S::TPT > 512)
+            w /= 2;
         return w;
     }
     static constexpr int W       = width(); // transforms per workgroup
