@@ -451,8 +451,9 @@ enum kernel_role : int
     role_direct = 0, // transform g is batch item g; contiguous block load and store
     role_col    = 1, // four-step pass A: column transforms, twiddled, into the intermediate
     role_row    = 2, // four-step pass B: row transforms of the intermediate, transposed store
-    role_pass   = 3  // one HBM pass of a multi-pass Stockham transform: W consecutive butterflies of
+    role_pass   = 3, // one HBM pass of a multi-pass Stockham transform: W consecutive butterflies of
                      // radix 2^L, twiddled inputs, outputs at the autosort positions
+    role_conv   = 4  // a transform inside another kernel (one per workgroup, whole-length LDS)
 };
 
 template <typename T>

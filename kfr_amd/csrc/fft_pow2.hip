@@ -71,12 +71,14 @@ __host__ __device__ constexpr int pass_twoff(int L, int le, int p)
     return o;
 }
 
-template <typename T, int L>
+// EX: points per thread as a multiple of the radix of the passes (2 gives every thread two
+// butterflies per pass and halves the threads per transform)
+template <typename T, int L, int EX = 1>
 struct shape
 {
     static constexpr int N   = 1 << L;
     static constexpr int LE  = pass_le(L, log_e<T>);
-    static constexpr int E   = 1 << LE;
+    static constexpr int E   = (1 << LE) * EX;
     static constexpr int TPT = N / E;
     static constexpr int NP  = pass_count(L, log_e<T>);
     static constexpr int lr(int p) { return pass_lr(L, log_e<T>, p); }
@@ -85,36 +87,54 @@ struct shape
     static constexpr int twcount = pass_twoff(L, log_e<T>, pass_count(L, log_e<T>));
 };
 
-// the values one thread holds
-template <typename T, int L>
-using regs = cx<T>[shape<T, L>::E];
-
 template <typename T, int L, int ROLE>
 struct geom
 {
-    using S = shape<T, L>;
+    using S1 = shape<T, L, 1>;
     static constexpr int width()
     {
+        if (ROLE == role_conv)
+            return 1;
         if (ROLE == role_direct)
-            return S::TPT >= 256 ? 1 : 256 / S::TPT;
+            return S1::TPT >= 256 ? 1 : 256 / S1::TPT;
         int w = 16;
-        while (w > 1 && (w * S::N > lds_budget<T> || w * S::TPT > 1024))
+        while (w > 1 && (w * S1::N > lds_budget<T> || w * S1::TPT > 1024))
             w /= 2;
         // runs longer than a 128-byte line buy nothing, two workgroups per CU do (measured for f64 at
         // 2^9 points: 512 threads and 128-byte runs beat 1024 threads and 256-byte runs)
-        if (ROLE != role_pass && w * (int)sizeof(cx<T>) > 128 && w * S::TPT > 512)
+        if (ROLE != role_pass && w * (int)sizeof(cx<T>) > 128 && w * S1::TPT > 512)
             w /= 2;
         return w;
     }
-    static constexpr int W       = width(); // transforms per workgroup
-    static constexpr int BS      = S::TPT * W;
-    static constexpr int LD      = ROLE == role_row ? S::N + 1 : S::N; // LDS distance between transforms
-    static constexpr int logical = ROLE == role_row ? W * LD : W * S::N;
+    static constexpr int W = width(); // transforms per workgroup
+    // Workgroups that would need more than 512 threads (and so more than 64 KiB of LDS) give every
+    // thread twice the points and exchange through LDS in two halves: same runs in HBM, half the
+    // threads and half the LDS, so two workgroups share a CU.
+    static constexpr int ex()
+    {
+        if (ROLE == role_pass || ROLE == role_conv || S1::LE < log_e<T>)
+            return 1;
+        if (ROLE == role_direct)
+            return S1::TPT >= 512 ? 2 : 1;
+        return W * S1::TPT > 512 ? 2 : 1;
+    }
+    static constexpr int EX = ex();
+    static constexpr int H  = EX; // LDS exchange rounds
+    using S                 = shape<T, L, EX>;
+    static constexpr int BS = S::TPT * W;
+    static constexpr int NH = S::N / H; // positions per exchange round
+    static constexpr int LD = ROLE == role_row ? NH + 1 : NH; // LDS distance between transforms
+    static constexpr int logical = ROLE == role_row ? W * LD : W * NH;
     static constexpr size_t lds_bytes = (size_t)(logical + logical / 16 + 1) * sizeof(cx<T>);
     // short transforms (under 16 threads each) load and store through LDS in whole-workgroup runs
     static constexpr bool staged = ROLE == role_direct && S::TPT < 16;
     static_assert(BS <= 1024, "workgroup too large");
+    static_assert(H == 1 || (!staged && ROLE != role_pass), "two-round exchange needs whole-thread transforms");
 };
+
+// the values one thread holds
+template <typename T, int L, int ROLE>
+using regs = cx<T>[geom<T, L, ROLE>::S::E];
 
 // LDS slot of element i of the workgroup's transform tb: columns are interleaved (consecutive lanes
 // on consecutive columns), other roles keep each transform contiguous; one padding slot per 16
@@ -353,11 +373,11 @@ __device__ __forceinline__ void lds_barrier()
 
 // pass P of the schedule on the registers v (strided set in, strided set out after the exchange)
 template <typename T, int L, bool INV, int ROLE, int PREC, int P>
-__device__ __forceinline__ void run_pass(regs<T, L>& v, cx<T>* lds, int tb, int t, const cx<T>* tw,
+__device__ __forceinline__ void run_pass(regs<T, L, ROLE>& v, cx<T>* lds, int tb, int t, const cx<T>* tw,
                                          const mfma16& K)
 {
-    using S          = shape<T, L>;
     using G          = geom<T, L, ROLE>;
+    using S          = typename G::S;
     constexpr int E  = S::E;
     constexpr int TP = S::TPT;
     constexpr int R  = 1 << S::lr(P);
@@ -395,22 +415,46 @@ __device__ __forceinline__ void run_pass(regs<T, L>& v, cx<T>* lds, int tb, int 
     }
     if constexpr (P + 1 < S::NP)
     {
-        // output r of butterfly j goes to (j - j mod Ns)*R + j mod Ns + r*Ns
+        // output r of butterfly j goes to (j - j mod Ns)*R + j mod Ns + r*Ns. With H rounds, round h
+        // moves positions [h*N/H, (h+1)*N/H): on a pass before the last these are exactly the outputs
+        // of butterflies m in [h*M/H, (h+1)*M/H) (the top bits of the position are those of j / Ns)
+        // and the inputs k in [h*E/H, (h+1)*E/H) of the next pass.
+        // Round h reads into the registers it has just written out (slots m + r*M of its butterflies),
+        // so the values of later rounds are never overwritten; a compile-time permutation then
+        // restores the order v[k] = x[t + k*TPT].
+        constexpr int H  = G::H;
+        constexpr int MH = M / H;
 #pragma unroll
-        for (int m = 0; m < M; ++m)
+        for (int h = 0; h < H; ++h)
         {
-            const int j    = t + m * TP;
-            const int kt   = j & (NS - 1);
-            const int base = (j - kt) * R + kt;
 #pragma unroll
-            for (int r = 0; r < R; ++r)
-                lds[laddr<ROLE, G::W, G::LD>(tb, base + r * NS)] = v[m + r * M];
+            for (int m = h * MH; m < (h + 1) * MH; ++m)
+            {
+                const int j    = t + m * TP;
+                const int kt   = j & (NS - 1);
+                const int base = (j - kt) * R + kt - h * G::NH;
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    lds[laddr<ROLE, G::W, G::LD>(tb, base + r * NS)] = v[m + r * M];
+            }
+            lds_barrier();
+#pragma unroll
+            for (int i = 0; i < E / H; ++i)
+                v[h * MH + i % MH + (i / MH) * M] = lds[laddr<ROLE, G::W, G::LD>(tb, t + (h * (E / H) + i) * TP - h * G::NH)];
+            lds_barrier();
         }
-        lds_barrier();
+        if constexpr (H > 1)
+        {
+            cx<T> u[E];
 #pragma unroll
-        for (int k = 0; k < E; ++k)
-            v[k] = lds[laddr<ROLE, G::W, G::LD>(tb, t + k * TP)];
-        lds_barrier();
+            for (int h = 0; h < H; ++h)
+#pragma unroll
+                for (int i = 0; i < E / H; ++i)
+                    u[h * (E / H) + i] = v[h * MH + i % MH + (i / MH) * M];
+#pragma unroll
+            for (int k = 0; k < E; ++k)
+                v[k] = u[k];
+        }
         run_pass<T, L, INV, ROLE, PREC, P + 1>(v, lds, tb, t, tw, K);
     }
 }
@@ -427,8 +471,8 @@ __device__ __forceinline__ int mid_scale_exp(float mx)
 template <typename T, int L, bool INV, int ROLE, int PREC>
 __global__ void __launch_bounds__((geom<T, L, ROLE>::BS)) k_pow2(args<T> a)
 {
-    using S          = shape<T, L>;
     using G          = geom<T, L, ROLE>;
+    using S          = typename G::S;
     constexpr int N  = S::N;
     constexpr int E  = S::E;
     constexpr int TP = S::TPT;
@@ -649,24 +693,33 @@ __global__ void __launch_bounds__((geom<T, L, ROLE>::BS)) k_pow2(args<T> a)
     }
     else
     {
-#pragma unroll
-        for (int k = 0; k < E; ++k)
-            lds[laddr<ROLE, W, G::LD>(tb, t + k * TP)] = v[k];
-        lds_barrier();
-        // transposed store: row index fastest, so lanes write W consecutive outputs k2 + n2*k1
+        // transposed store: row index fastest, so lanes write W consecutive outputs k2 + n2*k1; with
+        // H rounds, round h moves rows k1 in [h*N/H, (h+1)*N/H) (values k in [h*E/H, (h+1)*E/H))
+        constexpr int H    = G::H;
         const int tb2      = tid % W;
         const int i0       = tid / W;
         const long long g2 = (long long)blk * W + tb2;
-        if (g2 < a.count)
-        {
-            const long long bl   = g2 >> a.lo;
-            const long long k2   = g2 & ((1ll << a.lo) - 1);
-            const long long base = (bl << (a.lo + L)) + k2;
+        const long long bl   = g2 >> a.lo;
+        const long long k2   = g2 & ((1ll << a.lo) - 1);
+        const long long base = (bl << (a.lo + L)) + k2;
 #pragma unroll
-            for (int k = 0; k < E; ++k)
+        for (int h = 0; h < H; ++h)
+        {
+            if (h > 0)
+                lds_barrier();
+#pragma unroll
+            for (int k = h * (E / H); k < (h + 1) * (E / H); ++k)
+                lds[laddr<ROLE, W, G::LD>(tb, t + k * TP - h * G::NH)] = v[k];
+            lds_barrier();
+            if (g2 < a.count)
             {
-                const int i = i0 + k * TP;
-                st_elem<T, PREC>(a.out, base + ((long long)i << a.lo), lds[laddr<ROLE, W, G::LD>(tb2, i)]);
+#pragma unroll
+                for (int k = h * (E / H); k < (h + 1) * (E / H); ++k)
+                {
+                    const int i = i0 + k * TP;
+                    st_elem<T, PREC>(a.out, base + ((long long)i << a.lo),
+                                     lds[laddr<ROLE, W, G::LD>(tb2, i - h * G::NH)]);
+                }
             }
         }
     }
@@ -729,6 +782,20 @@ int col_width(int L)
         return 1;
 }
 
+// threads per transform of the one-workgroup kernel of 2^L points
+template <typename T, int LO, int HI>
+int direct_threads(int L)
+{
+    if constexpr (LO <= HI)
+    {
+        if (L == LO)
+            return geom<T, LO, role_direct>::S::TPT;
+        return direct_threads<T, LO + 1, HI>(L);
+    }
+    else
+        return 1;
+}
+
 // pass tables of 2^L points (layout of pass_twoff), FP64-generated
 template <typename T>
 void gen_pass_tables(int L, cx<T>* tw, hipStream_t s)
@@ -782,7 +849,7 @@ struct conv_os_args
 template <typename T, int L>
 __global__ void __launch_bounds__((shape<T, L>::TPT)) k_conv_os(conv_os_args<T> a)
 {
-    using S          = shape<T, L>;
+    using S          = typename geom<T, L, role_conv>::S;
     constexpr int E  = S::E;
     constexpr int TP = S::TPT;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -807,7 +874,7 @@ __global__ void __launch_bounds__((shape<T, L>::TPT)) k_conv_os(conv_os_args<T> 
         v[k].re = (pa >= 0 && pa < a.nu) ? u[pa] : T(0);
         v[k].im = (pb >= 0 && pb < a.nu) ? u[pb] : T(0);
     }
-    run_pass<T, L, false, role_direct, prec_full, 0>(v, lds, 0, t, a.tw, K);
+    run_pass<T, L, false, role_conv, prec_full, 0>(v, lds, 0, t, a.tw, K);
     // v[k] = Z[t + k*TPT]
     const cx<T>* H = a.H + item * a.hd;
 #pragma unroll
@@ -817,7 +884,7 @@ __global__ void __launch_bounds__((shape<T, L>::TPT)) k_conv_os(conv_os_args<T> 
         const cx<T> p = cmul(v[k], h);
         v[k]          = { p.re * a.scale, p.im * a.scale };
     }
-    run_pass<T, L, true, role_direct, prec_full, 0>(v, lds, 0, t, a.tw, K);
+    run_pass<T, L, true, role_conv, prec_full, 0>(v, lds, 0, t, a.tw, K);
     T* y = a.y + item * a.dy;
 #pragma unroll
     for (int k = 0; k < E; ++k)
@@ -857,7 +924,7 @@ struct xcorr_args
 template <typename T, int L>
 __global__ void __launch_bounds__((shape<T, L>::TPT)) k_conv_xcorr(xcorr_args<T> a)
 {
-    using S          = shape<T, L>;
+    using S          = typename geom<T, L, role_conv>::S;
     constexpr int N  = S::N;
     constexpr int E  = S::E;
     constexpr int TP = S::TPT;
@@ -882,17 +949,17 @@ __global__ void __launch_bounds__((shape<T, L>::TPT)) k_conv_xcorr(xcorr_args<T>
         v[k].re           = (n < a.P && p < a.nx) ? x[p] : T(0);
         v[k].im           = p < a.ng ? g[p] : T(0);
     }
-    run_pass<T, L, false, role_direct, prec_full, 0>(v, lds, 0, t, a.tw, K);
+    run_pass<T, L, false, role_conv, prec_full, 0>(v, lds, 0, t, a.tw, K);
     // v[k] = Z[t + k*TPT]; fetch Z[-n] through LDS
 #pragma unroll
     for (int k = 0; k < E; ++k)
-        lds[laddr<role_direct, 1, N>(0, t + k * TP)] = v[k];
+        lds[laddr<role_conv, 1, N>(0, t + k * TP)] = v[k];
     lds_barrier();
 #pragma unroll
     for (int k = 0; k < E; ++k)
     {
         const int n      = t + k * TP;
-        const cx<T> zm   = lds[laddr<role_direct, 1, N>(0, (N - n) & (N - 1))];
+        const cx<T> zm   = lds[laddr<role_conv, 1, N>(0, (N - n) & (N - 1))];
         const cx<T> z    = v[k];
         const cx<T> X    = { T(0.5) * (z.re + zm.re), T(0.5) * (z.im - zm.im) };
         // G = (z - conj zm) / (2i)
@@ -902,7 +969,7 @@ __global__ void __launch_bounds__((shape<T, L>::TPT)) k_conv_xcorr(xcorr_args<T>
         v[k] = { (G.re * X.re + G.im * X.im) * a.scale, (G.im * X.re - G.re * X.im) * a.scale };
     }
     lds_barrier();
-    run_pass<T, L, true, role_direct, prec_full, 0>(v, lds, 0, t, a.tw, K);
+    run_pass<T, L, true, role_conv, prec_full, 0>(v, lds, 0, t, a.tw, K);
     T* out = a.part + w * a.nh;
 #pragma unroll
     for (int k = 0; k < E; ++k)
@@ -1251,7 +1318,7 @@ std::string pow2_fft<T>::describe() const
     }
     if (!p->four)
         return "pow2 radices=" + radices(p->L) + " threads/transform=" +
-               std::to_string((1 << p->L) >> p2::pass_le(p->L, le)) + pr;
+               std::to_string(p2::direct_threads<T, 1, p2::max_log<T>>(p->L)) + pr;
     return "pow2 four-step n1=" + std::to_string(1ll << p->L1) + " n2=" + std::to_string(1ll << p->L2) +
            " radices=" + radices(p->L2) + "|" + radices(p->L1) + " chunk=" + std::to_string(p->chunk) + pr +
            (p->prec != p2::prec_full ? " intermediate=f16 scaled per " + std::to_string(1 << p->gshift) + " columns"
