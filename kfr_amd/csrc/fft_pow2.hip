@@ -980,22 +980,20 @@ __global__ void __launch_bounds__((shape<T, L>::TPT)) k_conv_xcorr(xcorr_args<T>
     }
 }
 
-// dh[b][j] = sum over the segments (and over the items when the filter is shared) of the partials
+// out[b][j] = sum over s < nseg of part[b][s][j] (a shared filter sums the items in a second call)
 template <typename T>
-__global__ void k_conv_xcorr_sum(T* dh, const T* part, long long nh, long long nseg, long long items,
-                                 bool shared)
+__global__ void k_conv_xcorr_sum(T* out, const T* part, long long nh, long long nseg, long long items)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long outs = shared ? nh : items * nh;
-    if (i >= outs)
+    if (i >= items * nh)
         return;
-    const long long b0 = shared ? 0 : i / nh, b1 = shared ? items : b0 + 1;
-    const long long j  = shared ? i : i - b0 * nh;
-    T acc              = T(0);
-    for (long long b = b0; b < b1; ++b)
-        for (long long s = 0; s < nseg; ++s)
-            acc += part[(b * nseg + s) * nh + j];
-    dh[i] = acc;
+    const long long b = i / nh;
+    const long long j = i - b * nh;
+    const T* p        = part + b * nseg * nh + j;
+    T acc             = T(0);
+    for (long long s = 0; s < nseg; ++s)
+        acc += p[s * nh];
+    out[i] = acc;
 }
 
 // complex filter buffer (f[n], 0) for n < nf, zero to N; f = h reversed when rev
@@ -1335,7 +1333,7 @@ struct os_conv<T>::impl
 {
     size_t nx = 0, nh = 0;
     int L     = 0;
-    dev_mem tw, hz, H, part;
+    dev_mem tw, hz, H, part, sum;
     std::unique_ptr<pow2_fft<T>> fft; // filter spectra
 };
 
@@ -1464,11 +1462,23 @@ void os_conv<T>::backward(T* dx, T* dh, const T* g, const T* x, const T* h, long
         a.part = q.part.template as<T>();
         if (a.count > 0)
             p2::conv_xcorr_any<T, p2::conv_min_log>(q.L, a, s);
-        const bool shared    = h_dist == 0;
-        const long long outs = shared ? (long long)q.nh : batch * (long long)q.nh;
-        hipLaunchKernelGGL(p2::k_conv_xcorr_sum<T>, dim3(nblocks(outs, 256)), dim3(256), 0, s, dh, a.part,
-                           (long long)q.nh, a.nseg, batch, shared);
+        // per item over its segments, then (shared filter) over the items
+        const long long nh = (long long)q.nh;
+        T* per_item        = dh;
+        if (h_dist == 0)
+        {
+            q.sum.ensure((size_t)(batch * nh) * sizeof(T));
+            per_item = q.sum.template as<T>();
+        }
+        hipLaunchKernelGGL(p2::k_conv_xcorr_sum<T>, dim3(nblocks(batch * nh, 256)), dim3(256), 0, s, per_item,
+                           a.part, nh, a.nseg, batch);
         hip_check(hipGetLastError(), "gradient reduction");
+        if (h_dist == 0)
+        {
+            hipLaunchKernelGGL(p2::k_conv_xcorr_sum<T>, dim3(nblocks(nh, 256)), dim3(256), 0, s, dh, per_item, nh,
+                               batch, 1ll);
+            hip_check(hipGetLastError(), "gradient reduction");
+        }
     }
 }
 

@@ -269,3 +269,29 @@ def test_overlap_save_lengths(dt, nx, nh):
     ec = oracle.rel_rms(kfr_amd.correlate(a, b), refc)
     print(f"overlap-save {np.dtype(dt).name} nx={nx} nh={nh} conv={e:.3e} corr={ec:.3e}", flush=True)
     assert e < tol and ec < tol
+
+
+def test_conv_shared_filter_gradient_many_items():
+    """a filter shared by 96 signals: its gradient sums 96 x (segments) partial correlations"""
+    torch = pytest.importorskip("torch")
+    from kfr_amd.autograd import fft_convolve
+    B, nx, nh = 96, 1 << 16, 1000
+    g = torch.Generator(device="cuda")
+    g.manual_seed(9)
+    x = torch.randn(B, nx, device="cuda", generator=g).requires_grad_(True)
+    h = torch.randn(nh, device="cuda", generator=g).requires_grad_(True)
+    y = fft_convolve(x, h)
+    gout = torch.randn(B, nx + nh - 1, device="cuda", generator=g)
+    y.backward(gout)
+    S = 1 << 17
+    xr = x.detach().double().requires_grad_(True)
+    hr = h.detach().double().requires_grad_(True)
+    yr = torch.fft.irfft(torch.fft.rfft(xr, S) * torch.fft.rfft(hr, S), S)[:, : nx + nh - 1]
+    yr.backward(gout.double())
+
+    def rel(a, b):
+        return float((a.detach().double() - b.detach()).norm() / b.detach().norm())
+
+    ey, ex, eh = rel(y, yr), rel(x.grad, xr.grad), rel(h.grad, hr.grad)
+    print(f"shared filter B={B}: rel_rms y={ey:.3e} dx={ex:.3e} dh={eh:.3e}", flush=True)
+    assert ey < 1e-6 and ex < 1e-6 and eh < 1e-5
