@@ -87,7 +87,7 @@ struct shape
     static constexpr int twcount = pass_twoff(L, log_e<T>, pass_count(L, log_e<T>));
 };
 
-template <typename T, int L, int ROLE>
+template <typename T, int L, int ROLE, int PREC = 0>
 struct geom
 {
     using S1 = shape<T, L, 1>;
@@ -112,7 +112,9 @@ struct geom
     // threads and half the LDS, so two workgroups share a CU.
     static constexpr int ex()
     {
-        if (ROLE == role_pass || ROLE == role_conv || S1::LE < log_e<T>)
+        // measured: column kernels (whose strided loads, twiddles and stores hold the most registers)
+        // and the half-precision kernels (MFMA operands) run faster with the full workgroup
+        if (ROLE == role_pass || ROLE == role_conv || ROLE == role_col || PREC != 0 || S1::LE < log_e<T>)
             return 1;
         if (ROLE == role_direct)
             return S1::TPT >= 512 ? 2 : 1;
@@ -133,8 +135,8 @@ struct geom
 };
 
 // the values one thread holds
-template <typename T, int L, int ROLE>
-using regs = cx<T>[geom<T, L, ROLE>::S::E];
+template <typename T, int L, int ROLE, int PREC>
+using regs = cx<T>[geom<T, L, ROLE, PREC>::S::E];
 
 // LDS slot of element i of the workgroup's transform tb: columns are interleaved (consecutive lanes
 // on consecutive columns), other roles keep each transform contiguous; one padding slot per 16
@@ -373,10 +375,10 @@ __device__ __forceinline__ void lds_barrier()
 
 // pass P of the schedule on the registers v (strided set in, strided set out after the exchange)
 template <typename T, int L, bool INV, int ROLE, int PREC, int P>
-__device__ __forceinline__ void run_pass(regs<T, L, ROLE>& v, cx<T>* lds, int tb, int t, const cx<T>* tw,
+__device__ __forceinline__ void run_pass(regs<T, L, ROLE, PREC>& v, cx<T>* lds, int tb, int t, const cx<T>* tw,
                                          const mfma16& K)
 {
-    using G          = geom<T, L, ROLE>;
+    using G          = geom<T, L, ROLE, PREC>;
     using S          = typename G::S;
     constexpr int E  = S::E;
     constexpr int TP = S::TPT;
@@ -469,9 +471,9 @@ __device__ __forceinline__ int mid_scale_exp(float mx)
 }
 
 template <typename T, int L, bool INV, int ROLE, int PREC>
-__global__ void __launch_bounds__((geom<T, L, ROLE>::BS)) k_pow2(args<T> a)
+__global__ void __launch_bounds__((geom<T, L, ROLE, PREC>::BS)) k_pow2(args<T> a)
 {
-    using G          = geom<T, L, ROLE>;
+    using G          = geom<T, L, ROLE, PREC>;
     using S          = typename G::S;
     constexpr int N  = S::N;
     constexpr int E  = S::E;
@@ -728,7 +730,7 @@ __global__ void __launch_bounds__((geom<T, L, ROLE>::BS)) k_pow2(args<T> a)
 template <typename T, int L, bool INV, int ROLE, int PREC>
 void launch(args<T> a, long long count, hipStream_t s)
 {
-    using G                  = geom<T, L, ROLE>;
+    using G                  = geom<T, L, ROLE, PREC>;
     static const bool lds_ok = (allow_lds(k_pow2<T, L, INV, ROLE, PREC>, G::lds_bytes), true);
     (void)lds_ok;
     if (count <= 0)
