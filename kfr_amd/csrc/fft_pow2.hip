@@ -1169,9 +1169,25 @@ pow2_fft<T>::pow2_fft(size_t n, hipStream_t s, int prec) : p(new impl)
         p->chunk = (long long)std::max<size_t>(1, cap / (n * sizeof(cx<T>)));
         return;
     }
-    p->four              = true;
-    p->L2                = std::max(p2::min_split_log, L / 2);
-    p->L1                = L - p->L2;
+    p->four = true;
+    p->L2   = std::max(p2::min_split_log, L / 2);
+    // column length 2^L2 measured fastest per length (sweep of L2 over 7..14 on MI355X, full
+    // precision): the column and row kernels of 2^8..2^9 points run near half of peak, those of
+    // 2^10..2^12 points near a third, so uneven splits often win
+    if (prec == p2::prec_full)
+    {
+        static constexpr int best_f32[] = { 8, 8, 9, 9, 11, 8, 9, 10, 10 }; // L = 15..23
+        static constexpr int best_f64[] = { 7, 9, 9, 8, 8, 9, 10, 10, 10 }; // L = 14..22
+        if (sizeof(T) == 4 && L >= 15 && L <= 23)
+            p->L2 = best_f32[L - 15];
+        if (sizeof(T) == 8 && L >= 14 && L <= 22)
+            p->L2 = best_f64[L - 14];
+    }
+    if (const char* e = std::getenv("KFR_AMD_FOURSTEP_L2"))
+        p->L2 = std::atoi(e);
+    p->L2 = std::min(std::max(p->L2, std::max(p2::min_split_log, L - p2::max_log<T>)),
+                     std::min(p2::max_log<T>, L - p2::min_split_log));
+    p->L1 = L - p->L2;
     const int c2         = p2::pass_twoff(p->L2, le, p2::pass_count(p->L2, le));
     const int c1         = p2::pass_twoff(p->L1, le, p2::pass_count(p->L1, le));
     p->lbits             = (L + 1) / 2;
