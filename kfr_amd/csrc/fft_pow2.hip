@@ -1183,6 +1183,8 @@ pow2_fft<T>::pow2_fft(size_t n, hipStream_t s, int prec) : p(new impl)
         if (sizeof(T) == 8 && L >= 14 && L <= 22)
             p->L2 = best_f64[L - 14];
     }
+    else if (L == 20)
+        p->L2 = 12; // half precision, 2^20: measured 6% faster than 2^10 x 2^10
     if (const char* e = std::getenv("KFR_AMD_FOURSTEP_L2"))
         p->L2 = std::atoi(e);
     p->L2 = std::min(std::max(p->L2, std::max(p2::min_split_log, L - p2::max_log<T>)),
