@@ -244,6 +244,29 @@ def round_trip_config(torch, kfr_amd, stream, reps: int):
             "round_trip_rel_rms": float(f"{err:.3e}")}
 
 
+def md_config(torch, kfr_amd, stream, reps: int):
+    """Multidimensional complex-f32 transforms of 2^24 points: 4096 x 4096 and 256 x 256 x 256, forward,
+    Gaussian data (seed 7). Algorithmic bytes: one read and one write of the array (16 bytes per point)."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(7)
+    out = {}
+    for shape in ((4096, 4096), (256, 256, 256)):
+        n = int(np.prod(shape))
+        x = torch.randn(2 * n, device="cuda", generator=g)
+        y = torch.empty_like(x)
+        p = kfr_amd.dft_plan_md(shape, np.complex64)
+        t = _timed(torch, stream, lambda: p.execute_batch(y, x, 1, False, stream), reps)
+        xs = x[: 2 * n].view(torch.complex64).reshape(shape)
+        ref = torch.fft.fftn(xs.to(torch.complex128))
+        err = float((y.view(torch.complex64).reshape(shape).to(torch.complex128) - ref).norm() / ref.norm())
+        key = "x".join(str(v) for v in shape)
+        out[key] = {"ms": round(t, 4), "gbs": round(16 * n / (t * 1e-3) * 1e-9, 1),
+                    "frac": round(16 * n / (t * 1e-3) * 1e-9 / HBM_PEAK_GBS, 4),
+                    "gflops": round(5.0 * n * math.log2(n) / (t * 1e-3) * 1e-9, 1),
+                    "rel_rms_vs_fp64": float(f"{err:.3e}")}
+    return out
+
+
 def conv_config(torch, kfr_amd, stream, reps: int):
     """FFT convolution at full size: x = 128 signals of 2^20 f32 Gaussian samples (seed 5), h = 128
     Hann-windowed sincs of 4095 taps (cutoff 0.2 fs) plus 1e-3 N(0,1) (seed 6); y = x * h (length
@@ -512,10 +535,11 @@ def main() -> int:
     if not args.no_precision and not args.only:
         precision = precision_config(torch, kfr_amd, stream, max(3, args.steps))
 
-    round_trip = conv = None
+    round_trip = conv = md = None
     if not args.no_extra and not args.only:
         round_trip = round_trip_config(torch, kfr_amd, stream, max(3, args.steps))
         conv = conv_config(torch, kfr_amd, stream, max(3, args.steps))
+        md = md_config(torch, kfr_amd, stream, max(3, args.steps))
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -544,6 +568,7 @@ def main() -> int:
         "precision": precision,
         "round_trip_4096x256": round_trip,
         "conv": conv,
+        "md_c32": md,
         "top_kernels": top,
         "single_kernel_c2c_f32_frac": round(lds_bytes / (lds_t * 1e-3) * 1e-9 / HBM_PEAK_GBS, 4) if lds_t else None,
         "long_c2c_f32_frac": round(fs_bytes / (fs_t * 1e-3) * 1e-9 / HBM_PEAK_GBS, 4) if fs_t else None,

@@ -104,6 +104,22 @@ gpu_dft_md<T>::gpu_dft_md(const std::vector<size_t>& shape) : sh(shape)
 {
     check_shape(sh);
     total = product(sh);
+    // power-of-two axes of one-kernel length: the axes before the last run in place along their
+    // stride (one HBM round trip per axis, no transposes)
+    strided = (sh.back() & (sh.back() - 1)) == 0;
+    for (size_t i = 0; i + 1 < sh.size(); ++i)
+        strided = strided && (sh[i] == 1 || pow2_fft<T>::supports_axis(sh[i]));
+    if (strided)
+    {
+        if (sh.back() > 1)
+            plans[sh.back()].reset(new gpu_dft<T>(sh.back()));
+        hipStream_t s = library_stream();
+        for (size_t i = 0; i + 1 < sh.size(); ++i)
+            if (sh[i] > 1 && !axis.count(sh[i]))
+                axis[sh[i]].reset(new pow2_fft<T>(sh[i], s));
+        hip_check(hipStreamSynchronize(s), "multidimensional plan initialisation");
+        return;
+    }
     for (size_t n : sh)
         if (n > 1 && !plans.count(n))
             plans[n].reset(new gpu_dft<T>(n));
@@ -120,12 +136,49 @@ size_t gpu_dft_md<T>::size() const
 template <typename T>
 std::string gpu_dft_md<T>::schedule() const
 {
+    if (strided)
+    {
+        std::string t;
+        for (size_t i = 0; i + 1 < sh.size(); ++i)
+            t += "axis " + std::to_string(i) + " (" + std::to_string(sh[i]) + "): " +
+                 (sh[i] > 1 ? "in place along its stride, " + axis.at(sh[i])->describe() : std::string("identity")) +
+                 "\n";
+        const size_t n = sh.size() - 1;
+        return t + "axis " + std::to_string(n) + " (" + std::to_string(sh.back()) + "): " +
+               (sh.back() > 1 ? plans.at(sh.back())->schedule().text() : std::string("identity"));
+    }
     return axis_text(plans, sh, sh.size()) + "transpose after every axis";
 }
 
 template <typename T>
 void gpu_dft_md<T>::execute(void* out, const void* in, bool inverse, hipStream_t s)
 {
+    if (strided)
+    {
+        const size_t last = sh.back();
+        if (last > 1)
+            plans.at(last)->execute(out, in, (long long)(total / last), inverse, s);
+        else if (out != in)
+            hip_check(hipMemcpyAsync(out, in, total * sizeof(cx<T>), hipMemcpyDeviceToDevice, s), "copy");
+        int inner_log = 0;
+        for (size_t j = sh.size() - 1; j-- > 0;)
+        {
+            size_t t = sh[j + 1];
+            while (t > 1)
+            {
+                ++inner_log;
+                t >>= 1;
+            }
+            if (sh[j] > 1)
+            {
+                long long outer = 1;
+                for (size_t q = 0; q < j; ++q)
+                    outer *= (long long)sh[q];
+                axis.at(sh[j])->run_axis(inverse, out, out, inner_log, outer, s);
+            }
+        }
+        return;
+    }
     scratch.ensure(total * sizeof(cx<T>));
     std::vector<size_t> cur = sh; // current layout, last axis contiguous
     const void* src         = in;

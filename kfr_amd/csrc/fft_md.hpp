@@ -3,6 +3,7 @@
 #pragma once
 
 #include "fft_engine.hpp"
+#include "fft_pow2.hpp"
 
 #include <map>
 #include <memory>
@@ -29,6 +30,9 @@ private:
     std::vector<size_t> sh;
     size_t total = 0;
     std::map<size_t, std::unique_ptr<gpu_dft<T>>> plans; // one per distinct axis length
+    // power-of-two shapes: every axis but the last transformed in place along its stride
+    bool strided = false;
+    std::map<size_t, std::unique_ptr<pow2_fft<T>>> axis;
     dev_mem scratch;
 };
 

@@ -646,13 +646,16 @@ __global__ void __launch_bounds__((geom<T, L, ROLE, PREC>::BS)) k_pow2(args<T> a
         const long long col           = g & ((1ll << a.lo) - 1);
         const long long base          = (bl << (a.lo + L)) + col;
         const unsigned long long mask = (1ull << a.lbits) - 1;
-#pragma unroll
-        for (int k = 0; k < E; ++k)
+        if (a.ta) // four-step twiddles (absent when transforming one axis of a multidimensional array)
         {
-            const int k2               = t + k * TP;
-            const unsigned long long m = (unsigned long long)col * (unsigned long long)k2; // < n
-            const cx<T> w              = cmul(a.ta[m >> a.lbits], a.tb[m & mask]);
-            v[k]                       = ctw<INV>(v[k], w);
+#pragma unroll
+            for (int k = 0; k < E; ++k)
+            {
+                const int k2               = t + k * TP;
+                const unsigned long long m = (unsigned long long)col * (unsigned long long)k2; // < n
+                const cx<T> w              = cmul(a.ta[m >> a.lbits], a.tb[m & mask]);
+                v[k]                       = ctw<INV>(v[k], w);
+            }
         }
         if constexpr (PREC == prec_full)
         {
@@ -1310,6 +1313,30 @@ void pow2_fft<T>::run(bool inverse, const void* in, void* out, long long batch, 
             return p->template run<p2::prec_f16>(inverse, in, out, batch, s);
     }
     p->template run<p2::prec_full>(inverse, in, out, batch, s);
+}
+
+template <typename T>
+bool pow2_fft<T>::supports_axis(size_t n)
+{
+    const int L = p2::log2_exact(n);
+    return L >= p2::min_split_log && L <= p2::max_log<T>;
+}
+
+template <typename T>
+void pow2_fft<T>::run_axis(bool inverse, const void* in, void* out, int inner_log, long long outer, hipStream_t s)
+{
+    if (!supports_axis(1ull << p->L) || p->prec != p2::prec_full || p->four || p->multi)
+        throw std::logic_error("pow2_fft: this plan cannot transform a strided axis");
+    if (outer <= 0)
+        return;
+    p2::args<T> a;
+    a.in  = in;
+    a.out = out;
+    a.tw  = p->tw.template as<cx<T>>();
+    a.ta  = nullptr; // no four-step twiddles
+    a.lo  = inner_log;
+    constexpr int ML = p2::max_log<T>;
+    p2::launch_any<T, role_col, p2::prec_full, p2::min_split_log, ML>(inverse, p->L, a, outer << inner_log, s);
 }
 
 template <typename T>

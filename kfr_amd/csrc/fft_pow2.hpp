@@ -25,6 +25,11 @@ public:
     // batch transforms: in and out hold batch*n complex values each, item b at offset b*n; in == out
     // is allowed. No scaling (the inverse is unnormalised).
     void run(bool inverse, const void* in, void* out, long long batch, hipStream_t s);
+    // lengths whose transform along a strided axis runs in one kernel (2^7 .. the one-kernel maximum)
+    static bool supports_axis(size_t n);
+    // transforms along a strided axis, in == out allowed: element (o, i, c) at ((o*n + i) << inner_log) + c
+    // for o < outer, i < n, c < 2^inner_log (full precision, one-kernel lengths only)
+    void run_axis(bool inverse, const void* in, void* out, int inner_log, long long outer, hipStream_t s);
     std::string describe() const;
 
 private:

@@ -183,6 +183,13 @@ class dft_plan_md:
         check()
         return out
 
+    def execute_batch(self, out, in_, batch: int, inverse: bool = False, stream=None):
+        """`batch` arrays of device tensors, one after another, on `stream` (asynchronous)."""
+        r = getattr(lib(), f"kfr_amd_dft_execute_batch_{self.t}")(self._h, _ptr(out), _ptr(in_), batch,
+                                                                   int(inverse), _stream_ptr(stream))
+        check(r == 0)
+        return out
+
     def dump(self):
         getattr(lib(), f"kfr_dft_dump_{self.t}")(self._h)
 

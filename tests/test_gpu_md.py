@@ -13,7 +13,10 @@ from tests.common import tol_kfr
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(6, 8, 10), (32, 64), (3, 1, 5, 4), (101, 16), (1024, 128), (64, 64, 64), (7, 9, 11), (1, 30)]
+# power-of-two shapes whose leading axes have 2^7..2^14 points run those axes in place along their
+# stride: (1024, 128), (128, 256, 64), (256, 1, 512), (2048, 1024), (16384, 128) (f32; f64 stops at 2^13)
+SHAPES = [(6, 8, 10), (32, 64), (3, 1, 5, 4), (101, 16), (1024, 128), (64, 64, 64), (7, 9, 11), (1, 30),
+          (128, 256, 64), (256, 1, 512), (2048, 1024), (16384, 128)]
 
 
 def _bound(shape, dtype):
@@ -114,3 +117,21 @@ def test_md_dump_lists_every_axis(capfd):
     p.dump()
     out = capfd.readouterr().out
     assert "axis 0 (12)" in out and "axis 1 (16)" in out
+
+
+def test_md_strided_device_batch_and_schedule():
+    """device arrays through the batched entry point, two arrays in one call, both directions"""
+    torch = pytest.importorskip("torch")
+    shape = (512, 2048)
+    rng = np.random.default_rng(5)
+    x = (rng.standard_normal((2,) + shape) + 1j * rng.standard_normal((2,) + shape)).astype(np.complex64)
+    p = kfr_amd.dft_plan_md(shape, np.complex64)
+    xd = torch.from_numpy(x.view(np.float32).copy()).cuda()
+    yd = torch.empty_like(xd)
+    for inverse in (False, True):
+        p.execute_batch(yd, xd, 2, inverse)
+        torch.cuda.synchronize()
+        y = yd.cpu().numpy().view(np.complex64).reshape((2,) + shape)
+        xx = x.astype(np.complex128)
+        ref = np.fft.ifftn(xx, axes=(1, 2)) * (512 * 2048) if inverse else np.fft.fftn(xx, axes=(1, 2))
+        assert oracle.rel_rms(y, ref) <= _bound(shape, np.complex64)
