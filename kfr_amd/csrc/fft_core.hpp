@@ -453,7 +453,8 @@ enum kernel_role : int
     role_row    = 2, // four-step pass B: row transforms of the intermediate, transposed store
     role_pass   = 3, // one HBM pass of a multi-pass Stockham transform: W consecutive butterflies of
                      // radix 2^L, twiddled inputs, outputs at the autosort positions
-    role_conv   = 4  // a transform inside another kernel (one per workgroup, whole-length LDS)
+    role_conv   = 4, // a transform inside another kernel (one per workgroup, whole-length LDS)
+    role_block  = 5  // a transform inside another kernel whose LDS only the passes use
 };
 
 template <typename T>
