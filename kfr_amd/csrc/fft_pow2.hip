@@ -72,6 +72,15 @@ __host__ __device__ constexpr int pass_twoff(int L, int le, int p)
     return o;
 }
 
+// ceil(log2(v)) of a positive compile-time value
+__host__ __device__ constexpr int ilog2c(int v)
+{
+    int l = 0;
+    while ((1 << l) < v)
+        ++l;
+    return l;
+}
+
 // EX: points per thread as a multiple of the radix of the passes (2 gives every thread two
 // butterflies per pass and halves the threads per transform)
 template <typename T, int L, int EX = 1>
@@ -538,15 +547,29 @@ __global__ void __launch_bounds__((geom<T, L, ROLE, PREC>::BS)) k_pow2(args<T> a
         {
             if (a.ns_log > 0)
             {
-                // input r of butterfly j is multiplied by W_{Ns*R}^(r * (j mod Ns))
+                // input r = t + k*TPT of butterfly j is multiplied by W_{Ns*R}^(r * kt), kt = j mod Ns:
+                // W^(kt*t) times W^(kt*TPT*k), the latter built from the table values W^(kt*TPT*2^b)
+                // with one complex product per value (rounding depth at most log2(E) + 1)
                 const unsigned long long kt   = (unsigned long long)col & ((1ull << a.ns_log) - 1);
                 const unsigned long long mask = (1ull << a.lbits) - 1;
+                const unsigned long long nsr  = (1ull << (a.ns_log + L)) - 1; // Ns*R - 1
+                auto table = [&](unsigned long long m) { return cmul(a.ta[m >> a.lbits], a.tb[m & mask]); };
+                constexpr int LEE = ilog2c(E);
+                cx<T> pw[LEE > 0 ? LEE : 1];
+#pragma unroll
+                for (int b = 0; b < LEE; ++b)
+                    pw[b] = table((kt * (unsigned long long)(TP << b)) & nsr);
+                cx<T> w[E];
+                w[0] = table(kt * (unsigned long long)t);
+#pragma unroll
+                for (int k = 1; k < E; ++k)
+                {
+                    const int hb = ilog2c(k + 1) - 1; // top bit of k
+                    w[k]         = cmul(w[k - (1 << hb)], pw[hb]);
+                }
 #pragma unroll
                 for (int k = 0; k < E; ++k)
-                {
-                    const unsigned long long m = kt * (unsigned long long)(t + k * TP); // < Ns*R
-                    v[k]                       = ctw<INV>(v[k], cmul(a.ta[m >> a.lbits], a.tb[m & mask]));
-                }
+                    v[k] = ctw<INV>(v[k], w[k]);
             }
         }
     }
