@@ -684,14 +684,26 @@ __global__ void __launch_bounds__((geom<T, L, ROLE, PREC>::BS)) k_pow2(args<T> a
         const unsigned long long mask = (1ull << a.lbits) - 1;
         if (a.ta) // four-step twiddles (absent when transforming one axis of a multidimensional array)
         {
+            // W_n^(col*k2), k2 = t + k*TPT: W^(col*t) times W^(col*TPT*k), the latter built from the
+            // table values W^(col*TPT*2^b) with one complex product per value
+            const unsigned long long nm = (1ull << (a.lo + L)) - 1; // n - 1
+            auto table = [&](unsigned long long m) { return cmul(a.ta[m >> a.lbits], a.tb[m & mask]); };
+            constexpr int LEE = ilog2c(E);
+            cx<T> pw[LEE > 0 ? LEE : 1];
+#pragma unroll
+            for (int b = 0; b < LEE; ++b)
+                pw[b] = table(((unsigned long long)col * (unsigned long long)(TP << b)) & nm);
+            cx<T> w[E];
+            w[0] = table((unsigned long long)col * (unsigned long long)t);
+#pragma unroll
+            for (int k = 1; k < E; ++k)
+            {
+                const int hb = ilog2c(k + 1) - 1; // top bit of k
+                w[k]         = cmul(w[k - (1 << hb)], pw[hb]);
+            }
 #pragma unroll
             for (int k = 0; k < E; ++k)
-            {
-                const int k2               = t + k * TP;
-                const unsigned long long m = (unsigned long long)col * (unsigned long long)k2; // < n
-                const cx<T> w              = cmul(a.ta[m >> a.lbits], a.tb[m & mask]);
-                v[k]                       = ctw<INV>(v[k], w);
-            }
+                v[k] = ctw<INV>(v[k], w[k]);
         }
         if constexpr (PREC == prec_full)
         {
