@@ -147,6 +147,15 @@ struct margs
     unsigned nblk = 0;
 };
 
+// ceil(log2(v)) of a positive compile-time value
+__host__ __device__ constexpr int mlog2c(int v)
+{
+    int l = 0;
+    while ((1 << l) < v)
+        ++l;
+    return l;
+}
+
 // element e of this thread's transform in HBM: base + e*stride (col: a column; direct, row: a row)
 template <int N, bool INV, int ROLE, int P>
 __device__ __forceinline__ void mpass(const margs& a, cx<float>* lds, int tb, int t, bool valid, long long base,
@@ -218,7 +227,38 @@ __device__ __forceinline__ void mpass(const margs& a, cx<float>* lds, int tb, in
     else
     {
         // last pass: Ns*R = N and j < Ns, so output r of butterfly j is element j + r*Ns
+        // Column kernel: W_n^(col*e) for e = t + m*TPT + r*Ns is W^(col*t) * A_m * B_r with
+        // A_m = W^(col*m*TPT) and B_r = W^(col*r*Ns), each built from table values W^(col*TPT*2^b) and
+        // W^(col*Ns*2^b) with one complex product per value (instead of a two-level gather per element)
         const unsigned long long mask = (1ull << a.lbits) - 1;
+        [[maybe_unused]] cx<float> A[M], B[R];
+        if constexpr (ROLE == role_col)
+        {
+            auto table = [&](unsigned long long q) { return cmul(a.ta[q >> a.lbits], a.tb[q & mask]); };
+            const unsigned long long nm = (unsigned long long)a.n;
+            A[0] = table((unsigned long long)col * (unsigned long long)t);
+            cx<float> pa[mlog2c(M) > 0 ? mlog2c(M) : 1];
+#pragma unroll
+            for (int b = 0; (1 << b) < M; ++b)
+                pa[b] = table(((unsigned long long)col * (unsigned long long)(TP << b)) % nm);
+#pragma unroll
+            for (int m = 1; m < M; ++m)
+            {
+                const int hb = mlog2c(m + 1) - 1;
+                A[m]         = cmul(A[m - (1 << hb)], pa[hb]);
+            }
+            cx<float> pb[mlog2c(R) > 0 ? mlog2c(R) : 1];
+#pragma unroll
+            for (int b = 0; (1 << b) < R; ++b)
+                pb[b] = table(((unsigned long long)col * (unsigned long long)(NS << b)) % nm);
+            B[0] = { 1.f, 0.f };
+#pragma unroll
+            for (int r = 1; r < R; ++r)
+            {
+                const int hb = mlog2c(r + 1) - 1;
+                B[r]         = r == (1 << hb) ? pb[hb] : cmul(B[r - (1 << hb)], pb[hb]);
+            }
+        }
 #pragma unroll
         for (int m = 0; m < M; ++m)
         {
@@ -235,8 +275,7 @@ __device__ __forceinline__ void mpass(const margs& a, cx<float>* lds, int tb, in
                     {
                         if (valid)
                         {
-                            const unsigned long long q = (unsigned long long)col * (unsigned long long)e; // < n
-                            const cx<float> w          = cmul(a.ta[q >> a.lbits], a.tb[q & mask]);
+                            const cx<float> w = r == 0 ? A[m] : cmul(A[m], B[r]);
                             a.out[base + (long long)e * stride] = ctw<INV>(u[m][r], w);
                         }
                     }
