@@ -132,6 +132,10 @@ struct geom
     }
     static constexpr int EX = ex();
     static constexpr int H  = EX; // LDS exchange rounds
+    // pass twiddles built from powers (measured faster nearly everywhere; the two kernels below lose
+    // a wave per SIMD to the extra registers)
+    static constexpr bool tw_pow = !(ROLE == role_direct && sizeof(T) == 4 && L == 14) &&
+                                   !(ROLE == role_row && sizeof(T) == 8 && L == 10);
     using S                 = shape<T, L, EX>;
     static constexpr int BS = S::TPT * W;
     static constexpr int NH = S::N / H; // positions per exchange round
@@ -404,7 +408,22 @@ __device__ __forceinline__ void run_pass(regs<T, L, ROLE, PREC>& v, cx<T>* lds, 
             const int j  = t + m * TP;
             const int kt = j & (NS - 1);
             cx<T> w[R];
-            load_table<T, R>(tw + S::twoff(P) + kt * R, w);
+            if constexpr (G::tw_pow && R >= 8)
+            {
+                // W^(kt*r) from the row's entries at r = 2^b, one complex product per value: fewer
+                // loads than the whole row
+                const cx<T>* row = tw + S::twoff(P) + kt * R;
+                w[0]             = { T(1), T(0) };
+#pragma unroll
+                for (int b = 0; (1 << b) < R; ++b)
+                    w[1 << b] = row[1 << b];
+#pragma unroll
+                for (int r = 3; r < R; ++r)
+                    if ((r & (r - 1)) != 0)
+                        w[r] = cmul(w[r - (1 << (ilog2c(r + 1) - 1))], w[1 << (ilog2c(r + 1) - 1)]);
+            }
+            else
+                load_table<T, R>(tw + S::twoff(P) + kt * R, w);
 #pragma unroll
             for (int r = 1; r < R; ++r)
                 v[m + r * M] = ctw<INV>(v[m + r * M], w[r]);
