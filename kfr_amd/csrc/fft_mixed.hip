@@ -198,9 +198,27 @@ __device__ __forceinline__ void mpass(const margs& a, cx<float>* lds, int tb, in
             {
                 const int kt       = j % NS;
                 const cx<float>* w = a.tw + S::twoff(P) + kt * R;
+                if constexpr (R >= 4)
+                {
+                    // W^(kt*r) from the row's entries at r = 2^b, one complex product per value
+                    cx<float> p[R];
 #pragma unroll
-                for (int r = 1; r < R; ++r)
-                    u[m][r] = ctw<INV>(u[m][r], w[r]);
+                    for (int b = 0; (1 << b) < R; ++b)
+                        p[1 << b] = w[1 << b];
+#pragma unroll
+                    for (int r = 3; r < R; ++r)
+                        if ((r & (r - 1)) != 0)
+                            p[r] = cmul(p[r - (1 << (mlog2c(r + 1) - 1))], p[1 << (mlog2c(r + 1) - 1)]);
+#pragma unroll
+                    for (int r = 1; r < R; ++r)
+                        u[m][r] = ctw<INV>(u[m][r], p[r]);
+                }
+                else
+                {
+#pragma unroll
+                    for (int r = 1; r < R; ++r)
+                        u[m][r] = ctw<INV>(u[m][r], w[r]);
+                }
             }
             butterfly<R, INV>(u[m]);
         }
