@@ -1,16 +1,20 @@
 // Power-of-two C2C kernels specialised per length.
 //
 // A transform of N = 2^L points is split over TPT = N/E threads (E = 16 points per thread for f32,
-// 8 for f64). Every pass of the Stockham autosort schedule reads, in thread t, the strided set
-// x[t + k*TPT] (k < E), so the first pass takes its operands straight from HBM (coalesced across
-// threads), the passes exchange data through LDS, and the last pass, whose outputs are the same
-// strided set in natural order, writes straight to HBM. Radix-E passes (16 or 8) follow one
-// smaller power-of-two pass that absorbs L mod log2(E); with the radices, pass strides, table
-// offsets and loop bounds all compile-time constants, the butterflies run from registers.
+// 8 for f64, twice that where a workgroup would otherwise need more than 512 threads). Every pass of
+// the Stockham autosort schedule reads, in thread t, the strided set x[t + k*TPT] (k < E), so the
+// first pass takes its operands straight from HBM (coalesced across threads), the passes exchange
+// data through LDS (in two rounds of half the positions when E is doubled), and the last pass, whose
+// outputs are the same strided set in natural order, writes straight to HBM. Radix-16 (f32) or
+// radix-8 (f64) passes follow one smaller power-of-two pass that absorbs the remainder; with the
+// radices, pass strides, table offsets and loop bounds all compile-time constants, the butterflies
+// run from registers, and each pass builds its twiddles from the power-of-two entries of its table
+// row.
 //
 // Lengths beyond one workgroup's LDS use the four-step split n = n1*n2 with x[c + n1*i]:
 // kernel "col" transforms the n2-point columns (W columns per workgroup, consecutive lanes on
-// consecutive columns), multiplies by W_n^(c*k2) and writes the intermediate; kernel "row"
+// consecutive columns), multiplies by W_n^(c*k2) (built from table powers) and writes the
+// intermediate; kernel "row"
 // transforms the n1-point rows and stores them transposed through LDS, so both kernels move
 // W-element runs. Workgroups of both kernels are ordered so that neighbouring column (or row)
 // groups land on the same XCD and share its L2.
