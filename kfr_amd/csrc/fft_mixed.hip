@@ -105,8 +105,11 @@ struct mgeom
     {
         if (ROLE == role_direct)
             return S::TPT >= 256 ? 1 : 256 / S::TPT;
-        int w = 16;
-        while (w > 1 && (w * N > lds_values || w * S::TPT > 1024))
+        // row kernels: at most 512 threads (measured: two workgroups per CU without register spills beat
+        // the wider runs of 1024 threads for the 4000-point rows; column kernels keep the wider runs)
+        const int maxbs = ROLE == role_row ? 512 : 1024;
+        int w           = 16;
+        while (w > 1 && (w * N > lds_values || w * S::TPT > maxbs))
             w /= 2;
         return w;
     }
