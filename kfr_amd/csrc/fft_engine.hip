@@ -884,6 +884,8 @@ void gpu_conv<T>::forward(void* y, const void* x, const void* h, long long h_dis
                           hipStream_t s)
 {
     impl& q = *p;
+    if (batch <= 0)
+        return;
     if (q.os)
     {
         q.os->forward(static_cast<T*>(y), static_cast<const T*>(x), static_cast<const T*>(h), h_dist, batch,
@@ -903,6 +905,9 @@ void gpu_conv<T>::forward(void* y, const void* x, const void* h, long long h_dis
 }
 
 template <typename T>
+__global__ void k_zero(T* p, long long count);
+
+template <typename T>
 __global__ void k_sum_batch(T* out, const T* in, long long n, long long batch)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -919,6 +924,17 @@ void gpu_conv<T>::backward(void* dx, void* dh, const void* g, const void* x, con
                            long long batch, hipStream_t s)
 {
     impl& q = *p;
+    if (batch <= 0)
+    {
+        // no signals: the gradient of a shared filter is the empty sum
+        if (dh && h_dist == 0)
+        {
+            hipLaunchKernelGGL(k_zero<T>, dim3(nblocks((long long)q.nh, 256)), dim3(256), 0, s, static_cast<T*>(dh),
+                               (long long)q.nh);
+            hip_check(hipGetLastError(), "gradient clear");
+        }
+        return;
+    }
     if (q.os)
     {
         q.os->backward(static_cast<T*>(dx), static_cast<T*>(dh), static_cast<const T*>(g), static_cast<const T*>(x),

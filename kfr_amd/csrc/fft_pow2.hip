@@ -1541,6 +1541,8 @@ template <typename T>
 void os_conv<T>::forward(T* y, const T* x, const T* h, long long h_dist, long long batch, bool correlate,
                          hipStream_t s)
 {
+    if (batch <= 0)
+        return;
     impl& q              = *p;
     const long long nfil = h_dist == 0 ? 1 : batch;
     const cx<T>* H       = filter_spectra<T>(q, h, h_dist, nfil, correlate, s);
@@ -1552,6 +1554,8 @@ template <typename T>
 void os_conv<T>::backward(T* dx, T* dh, const T* g, const T* x, const T* h, long long h_dist, long long batch,
                           hipStream_t s)
 {
+    if (batch <= 0)
+        return;
     impl& q            = *p;
     const long long ny = (long long)(q.nx + q.nh - 1);
     const long long N  = 1ll << q.L;

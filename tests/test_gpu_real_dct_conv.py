@@ -295,3 +295,22 @@ def test_conv_shared_filter_gradient_many_items():
     ey, ex, eh = rel(y, yr), rel(x.grad, xr.grad), rel(h.grad, hr.grad)
     print(f"shared filter B={B}: rel_rms y={ey:.3e} dx={ex:.3e} dh={eh:.3e}", flush=True)
     assert ey < 1e-6 and ex < 1e-6 and eh < 1e-5
+
+
+def test_conv_empty_batch():
+    """no signals: nothing is written by the forward call, the adjoint of a shared filter is zero"""
+    torch = pytest.importorskip("torch")
+    L = kfr_amd.lib()
+    for nx, nh in ((1 << 16, 255), (64, 16)):  # overlap-save and full-length paths
+        c = L.kfr_amd_conv_create(0, nx, nh)
+        assert c
+        y = torch.full((8,), 3.0, device="cuda")
+        dh = torch.full((nh,), 5.0, device="cuda")
+        x = torch.zeros(8, device="cuda")
+        h = torch.zeros(nh, device="cuda")
+        assert L.kfr_amd_conv_forward(c, y.data_ptr(), x.data_ptr(), h.data_ptr(), nh, 0, 0, None) == 0
+        assert L.kfr_amd_conv_backward(c, None, dh.data_ptr(), x.data_ptr(), x.data_ptr(), h.data_ptr(), 0, 0,
+                                       None) == 0
+        torch.cuda.synchronize()
+        assert torch.all(y == 3.0) and torch.all(dh == 0.0)
+        L.kfr_amd_conv_delete(c)
