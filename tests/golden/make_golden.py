@@ -72,7 +72,7 @@ def main():
             sp[f"c_in_{tn}_{n}"] = x
             sp[f"c_fwd_{tn}_{n}"] = ref.dft(x, False)
             sp[f"c_inv_{tn}_{n}"] = ref.dft(x, True)
-        # configuration 1 shape, two of the 256 transforms
+        # two of the 256 transforms of the 4096-point batch workload
         x = oracle.complex_uniform(4096, (2, 4096), cdt)
         sp[f"cfg1_in_{tn}"] = x
         sp[f"cfg1_fwd_{tn}"] = ref.dft(x, False)
