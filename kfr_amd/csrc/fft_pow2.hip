@@ -111,8 +111,12 @@ struct geom
             return 1;
         if (ROLE == role_direct)
             return S1::TPT >= 256 ? 1 : 256 / S1::TPT;
+        // full-precision rows that would exceed the budget exchange in two rounds (ex() below), holding
+        // half of each row in LDS at a time: twice the rows per workgroup and twice the run length of
+        // the transposed store (measured 2-6% faster for rows of 2^11..2^13 points)
+        constexpr int grow = ROLE == role_row && PREC == 0 && S1::LE >= log_e<T> ? 2 : 1;
         int w = 16;
-        while (w > 1 && (w * S1::N > lds_budget<T> || w * S1::TPT > 1024))
+        while (w > 1 && (w * S1::N > grow * lds_budget<T> || w * S1::TPT > grow * 1024))
             w /= 2;
         // runs longer than a 128-byte line buy nothing, two workgroups per CU do (measured for f64 at
         // 2^9 points: 512 threads and 128-byte runs beat 1024 threads and 256-byte runs)
@@ -1253,7 +1257,7 @@ pow2_fft<T>::pow2_fft(size_t n, hipStream_t s, int prec) : p(new impl)
     if (prec == p2::prec_full)
     {
         static constexpr int best_f32[] = { 8, 8, 9, 10, 11, 8, 9, 10, 10 }; // L = 15..23
-        static constexpr int best_f64[] = { 7, 9, 9, 8, 9, 10, 9, 10, 11 }; // L = 14..22
+        static constexpr int best_f64[] = { 7, 9, 9, 8, 9, 8, 9, 10, 10 };  // L = 14..22
         if (sizeof(T) == 4 && L >= 15 && L <= 23)
             p->L2 = best_f32[L - 15];
         if (sizeof(T) == 8 && L >= 14 && L <= 22)
