@@ -76,3 +76,56 @@ def test_pow2_f64_four_step_round_trip():
     err = oracle.rel_rms(z, x)
     print(f"f64 2^24 round trip rel_rms={err:.3e}", flush=True)
     assert err < 1e-14
+
+
+def _check_plan(dtype, L, batch, seed):
+    torch = pytest.importorskip("torch")
+    n = 1 << L
+    x = oracle.complex_uniform(seed, (batch, n), dtype)
+    rdt = np.float32 if dtype == np.complex64 else np.float64
+    xd = torch.from_numpy(x.view(rdt).copy()).cuda()
+    yd = torch.empty_like(xd)
+    plan = kfr_amd.dft_plan(n, dtype)
+    bound = max(1e-6 if dtype == np.complex64 else 1e-14, tol_kfr(n, dtype))
+    worst = 0.0
+    for inverse in (False, True):
+        ref = fp64_dft(x.astype(np.complex128), inverse)
+        plan.execute_batch(yd, xd, batch, inverse)
+        torch.cuda.synchronize()
+        y = yd.cpu().numpy().view(dtype).reshape(batch, n)
+        worst = max(worst, max(oracle.rel_rms(y[b], ref[b]) for b in range(batch)))
+    return worst, bound
+
+
+# (dtype, log2 n): every four-step split that KFR_AMD_FOURSTEP_L2 can select, including the ones the
+# measured table does not pick (rows of 2^14 points, the longest columns)
+SPLIT_CASES = [(np.complex64, L) for L in (15, 18, 21, 22)] + [(np.complex128, L) for L in (14, 17, 20, 22)]
+
+
+@pytest.mark.parametrize("dtype,L", SPLIT_CASES, ids=lambda v: getattr(v, "__name__", str(v)))
+def test_pow2_every_fourstep_split(dtype, L, monkeypatch):
+    t = "f32" if dtype == np.complex64 else "f64"
+    ml = 14 if dtype == np.complex64 else 13
+    batch = 3 if L <= 18 else 1
+    for l2 in range(max(7, L - ml), min(ml, L - 7) + 1):
+        monkeypatch.setenv("KFR_AMD_FOURSTEP_L2", str(l2))
+        sched = kfr_amd.schedule(t, 1 << L)
+        assert f"n2={1 << l2} " in sched, sched
+        worst, bound = _check_plan(dtype, L, batch, 1300 + L + l2)
+        print(f"L={L} {t} L2={l2} worst={worst:.3e} {sched}", flush=True)
+        assert worst <= bound, (L, l2, worst)
+
+
+@pytest.mark.parametrize("dtype,L", [(np.complex64, L) for L in (15, 17, 18, 20)] +
+                         [(np.complex128, L) for L in (14, 16, 17, 19)],
+                         ids=lambda v: getattr(v, "__name__", str(v)))
+def test_pow2_multipass_below_threshold(dtype, L, monkeypatch):
+    """the multi-pass Stockham selected for shorter lengths through KFR_AMD_MULTIPASS_MIN_LOG
+    (passes of radix 2^5..2^8: 2^17 runs 2^5 x 2^6 x 2^6)"""
+    t = "f32" if dtype == np.complex64 else "f64"
+    monkeypatch.setenv("KFR_AMD_MULTIPASS_MIN_LOG", str(L))
+    sched = kfr_amd.schedule(t, 1 << L)
+    assert "multi-pass" in sched, sched
+    worst, bound = _check_plan(dtype, L, 3 if L <= 17 else 1, 1500 + L)
+    print(f"L={L} {t} worst={worst:.3e} {sched}", flush=True)
+    assert worst <= bound, (L, worst)
