@@ -43,7 +43,11 @@ namespace p2
 template <typename T>
 inline constexpr int log_e = sizeof(T) == 4 ? 4 : 3; // log2 points per thread
 template <typename T>
-inline constexpr int max_log = sizeof(T) == 4 ? 14 : 13; // longest single-kernel transform
+inline constexpr int max_log = sizeof(T) == 4 ? 14 : 13; // longest four-step factor
+// longest full-precision transform in one workgroup: one length beyond the four-step factors, with
+// two exchange rounds (half of the transform in LDS at a time, 1024 threads, 136 KiB)
+template <typename T>
+inline constexpr int direct_max_log = max_log<T> + 1;
 template <typename T>
 inline constexpr int lds_budget = sizeof(T) == 4 ? 16384 : 8192; // complex values per workgroup
 constexpr int min_split_log = 7;                                 // shortest four-step factor
@@ -142,7 +146,7 @@ struct geom
     static constexpr int H  = EX; // LDS exchange rounds
     // pass twiddles built from powers (measured faster nearly everywhere; the two kernels below lose
     // a wave per SIMD to the extra registers)
-    static constexpr bool tw_pow = !(ROLE == role_direct && sizeof(T) == 4 && L == 14) &&
+    static constexpr bool tw_pow = !(ROLE == role_direct && sizeof(T) == 4 && L >= 14) &&
                                    !(ROLE == role_row && sizeof(T) == 8 && L == 10);
     using S                 = shape<T, L, EX>;
     static constexpr int BS = S::TPT * W;
@@ -1193,7 +1197,7 @@ pow2_fft<T>::pow2_fft(size_t n, hipStream_t s, int prec) : p(new impl)
     const int le = p2::log_e<T>;
     p->L         = L;
     p->prec      = prec;
-    if (L <= p2::max_log<T>)
+    if (L <= (prec == p2::prec_full ? p2::direct_max_log<T> : p2::max_log<T>))
     {
         const int cnt = p2::pass_twoff(L, le, p2::pass_count(L, le));
         p->tw.reset((size_t)std::max(1, cnt) * sizeof(cx<T>));
@@ -1347,7 +1351,8 @@ void pow2_fft<T>::impl::run(bool inverse, const void* in, void* out, long long b
         a.in  = in;
         a.out = out;
         a.tw  = t;
-        p2::launch_any<T, role_direct, PREC, 1, ML>(inverse, L, a, batch, s);
+        constexpr int DL = PREC == p2::prec_full ? p2::direct_max_log<T> : ML;
+        p2::launch_any<T, role_direct, PREC, 1, DL>(inverse, L, a, batch, s);
         return;
     }
     const long long n = 1ll << L;
@@ -1440,7 +1445,7 @@ std::string pow2_fft<T>::describe() const
     }
     if (!p->four)
         return "pow2 radices=" + radices(p->L) + " threads/transform=" +
-               std::to_string(p2::direct_threads<T, 1, p2::max_log<T>>(p->L)) + pr;
+               std::to_string(p2::direct_threads<T, 1, p2::direct_max_log<T>>(p->L)) + pr;
     return "pow2 four-step n1=" + std::to_string(1ll << p->L1) + " n2=" + std::to_string(1ll << p->L2) +
            " radices=" + radices(p->L2) + "|" + radices(p->L1) + " chunk=" + std::to_string(p->chunk) + pr +
            (p->prec != p2::prec_full ? " intermediate=f16 scaled per " + std::to_string(1 << p->gshift) + " columns"

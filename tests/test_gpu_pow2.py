@@ -99,7 +99,7 @@ def _check_plan(dtype, L, batch, seed):
 
 # (dtype, log2 n): every four-step split that KFR_AMD_FOURSTEP_L2 can select, including the ones the
 # measured table does not pick (rows of 2^14 points, the longest columns)
-SPLIT_CASES = [(np.complex64, L) for L in (15, 18, 21, 22)] + [(np.complex128, L) for L in (14, 17, 20, 22)]
+SPLIT_CASES = [(np.complex64, L) for L in (16, 18, 21, 22)] + [(np.complex128, L) for L in (15, 17, 20, 22)]
 
 
 @pytest.mark.parametrize("dtype,L", SPLIT_CASES, ids=lambda v: getattr(v, "__name__", str(v)))
@@ -116,12 +116,12 @@ def test_pow2_every_fourstep_split(dtype, L, monkeypatch):
         assert worst <= bound, (L, l2, worst)
 
 
-@pytest.mark.parametrize("dtype,L", [(np.complex64, L) for L in (15, 17, 18, 20)] +
-                         [(np.complex128, L) for L in (14, 16, 17, 19)],
+@pytest.mark.parametrize("dtype,L", [(np.complex64, L) for L in (16, 17, 18, 20)] +
+                         [(np.complex128, L) for L in (15, 16, 17, 19)],
                          ids=lambda v: getattr(v, "__name__", str(v)))
 def test_pow2_multipass_below_threshold(dtype, L, monkeypatch):
-    """the multi-pass Stockham selected for shorter lengths through KFR_AMD_MULTIPASS_MIN_LOG
-    (passes of radix 2^5..2^8: 2^17 runs 2^5 x 2^6 x 2^6)"""
+    """the multi-pass Stockham selected through KFR_AMD_MULTIPASS_MIN_LOG for lengths above one
+    workgroup (passes of radix 2^5..2^8: 2^17 runs 2^5 x 2^6 x 2^6)"""
     t = "f32" if dtype == np.complex64 else "f64"
     monkeypatch.setenv("KFR_AMD_MULTIPASS_MIN_LOG", str(L))
     sched = kfr_amd.schedule(t, 1 << L)
