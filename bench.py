@@ -520,10 +520,11 @@ def main() -> int:
                     "traffic_source": f"profiles/{TRAFFIC_TABLE}" if tr is not None else None,
                     "copy_gbs": round(copy_gbs, 1) if copy_gbs else None,
                     "frac_of_copy": round(d["gbs"] / copy_gbs, 4) if copy_gbs else None}
-    fs = [p for p in per if p["kind"] == "c2c" and p["dtype"] == "f32" and p["n"] > (1 << 14)]
+    # f32 lengths up to 2^15 run in one workgroup per transform, longer ones in two or three HBM passes
+    fs = [p for p in per if p["kind"] == "c2c" and p["dtype"] == "f32" and p["n"] > (1 << 15)]
     fs_t = sum(p["ms"] for p in fs)
     fs_bytes = sum(alg_bytes("c2c", "f32", p["n"], p["batch"]) for p in fs)
-    lds = [p for p in per if p["kind"] == "c2c" and p["dtype"] == "f32" and p["n"] <= (1 << 14)]
+    lds = [p for p in per if p["kind"] == "c2c" and p["dtype"] == "f32" and p["n"] <= (1 << 15)]
     lds_t = sum(p["ms"] for p in lds)
     lds_bytes = sum(alg_bytes("c2c", "f32", p["n"], p["batch"]) for p in lds)
 

@@ -15,7 +15,7 @@ template <typename T>
 class pow2_fft
 {
 public:
-    // true for N = 2^L that one kernel (L <= 14 for f32, 13 for f64), the two-kernel four-step split
+    // true for N = 2^L that one kernel (L <= 15 for f32, 14 for f64), the two-kernel four-step split
     // (up to twice that) or the multi-pass Stockham over HBM (the longest lengths) covers
     static bool supports(size_t n);
     // prec: storage of the data in HBM, 0 = complex T, 1 = bfloat16 pairs, 2 = binary16 pairs (T = float,
