@@ -464,11 +464,11 @@ struct gpu_dft_real<T>::impl
             hip_check(hipGetLastError(), "real forward launch");
             return;
         }
-        // rounds of at most 512 MiB of half-length spectra (measured: R2C 2^24 x 16 10% and 30,720,000 x 8
-        // 3.5% faster than 256 MiB rounds, whose extra launches each pay a ramp-up and drain; 1 GiB gains
-        // nothing more)
+        // rounds of at most 1 GiB of half-length spectra, as the complex plans inside (measured: R2C
+        // 2^24 x 16 12% and 30,720,000 x 8 4.5% faster than 256 MiB rounds, whose extra launches each
+        // pay a ramp-up and drain)
         const long long per = (long long)csize;
-        long long cap       = (long long)512 << 20;
+        long long cap       = (long long)1 << 30;
         if (const char* e = std::getenv("KFR_AMD_REAL_CHUNK_MB"))
             cap = (long long)std::max(1, atoi(e)) << 20;
         const long long chunk = std::max<long long>(1, cap / (per * (long long)sizeof(cx<T>)));

@@ -531,7 +531,7 @@ mixed_fft<T>::mixed_fft(size_t n, hipStream_t s) : p(new impl)
     mx::gen_tables(p->i1, t + p->o_row, s);
     gen_twiddles(t + p->o_ta, nTA, (long long)n, nTB, 0, tw_linear, 0, s);
     gen_twiddles(t + p->o_tb, nTB, (long long)n, 1, 0, tw_linear, 0, s);
-    size_t cap = (size_t)512 << 20;
+    size_t cap = (size_t)1 << 30; // rounds of at most 1 GiB, as the power-of-two four-step
     if (const char* e = std::getenv("KFR_AMD_FOURSTEP_CHUNK_MB"))
         cap = (size_t)std::max(1, atoi(e)) << 20;
     // fewer than 2^31 columns or rows per launch (32-bit indices)
