@@ -127,3 +127,28 @@ def test_reconstruction_sine():
         y /= n
         d = np.abs(y - x)
         assert np.sqrt(np.mean(d ** 2)) < 1e-6 and d.max() < 1e-5, n
+
+
+@pytest.mark.parametrize("n", [30240, 59049, 78125, 3 * 5 ** 6 * 7])
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_generic_four_step_batch(n, dtype):
+    """lengths above one workgroup with factors 3, 5, 7: four-step over the generic kernels with odd
+    column and row counts, so workgroups straddle batch items and the last one is partial"""
+    torch = pytest.importorskip("torch")
+    batch = 3
+    x = oracle.complex_uniform(700 + n % 97, (batch, n), dtype)
+    rdt = np.float32 if dtype == np.complex64 else np.float64
+    xd = torch.from_numpy(x.view(rdt).copy()).cuda()
+    yd = torch.empty_like(xd)
+    plan = kfr_amd.dft_plan(n, dtype)
+    sched = kfr_amd.schedule("f32" if dtype == np.complex64 else "f64", n)
+    assert "four-step" in sched, sched
+    bound = max(1e-6 if dtype == np.complex64 else 1e-14, tol_kfr(n, dtype))
+    for inverse in (False, True):
+        ref = fp64_dft(x.astype(np.complex128), inverse)
+        plan.execute_batch(yd, xd, batch, inverse)
+        torch.cuda.synchronize()
+        y = yd.cpu().numpy().view(dtype).reshape(batch, n)
+        errs = [oracle.rel_rms(y[b], ref[b]) for b in range(batch)]
+        print(f"n={n} {np.dtype(dtype).name} inv={inverse} worst={max(errs):.3e} {sched}", flush=True)
+        assert max(errs) <= bound, (n, inverse, errs)
