@@ -1204,10 +1204,11 @@ pow2_fft<T>::pow2_fft(size_t n, hipStream_t s, int prec) : p(new impl)
         p2::gen_pass_tables<T>(L, p->tw.template as<cx<T>>(), s);
         return;
     }
-    // the longest lengths (measured: from 2^23): HBM passes of radix 2^5..2^8 with
-    // 16 butterflies per workgroup (128-byte runs in and out, small workgroups); shorter lengths: the
-    // two-kernel four-step, whose extra pass costs more than its wider workgroups
-    int mp_min = 23;
+    // the longest lengths (measured with 1 GiB rounds: from 2^24; at 2^23 the four-step is 10% (f32)
+    // and 2% (f64) faster): HBM passes of radix 2^5..2^8 with 16 butterflies per workgroup (128-byte
+    // runs in and out, small workgroups); shorter lengths: the two-kernel four-step, whose extra pass
+    // costs more than its wider workgroups
+    int mp_min = 24;
     if (const char* e = std::getenv("KFR_AMD_MULTIPASS_MIN_LOG"))
         mp_min = atoi(e);
     if (prec == p2::prec_full && L >= mp_min)
@@ -1260,7 +1261,7 @@ pow2_fft<T>::pow2_fft(size_t n, hipStream_t s, int prec) : p(new impl)
     // 2^10..2^12 points near a third, so uneven splits often win
     if (prec == p2::prec_full)
     {
-        static constexpr int best_f32[] = { 8, 8, 9, 10, 8, 8, 9, 10, 10 };  // L = 15..23
+        static constexpr int best_f32[] = { 8, 8, 9, 10, 8, 8, 9, 10, 11 };  // L = 15..23
         static constexpr int best_f64[] = { 7, 9, 9, 8, 9, 8, 9, 10, 10 };  // L = 14..22
         if (sizeof(T) == 4 && L >= 15 && L <= 23)
             p->L2 = best_f32[L - 15];
