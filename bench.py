@@ -297,7 +297,7 @@ def conv_config(torch, kfr_amd, stream, reps: int):
 
     def bwd():
         assert L.kfr_amd_conv_backward(plan, dx.data_ptr(), dh.data_ptr(), gout.data_ptr(), x.data_ptr(),
-                                       h.data_ptr(), nh, B, st) == 0
+                                       h.data_ptr(), nh, B, 0, st) == 0
 
     tf = _timed(torch, stream, fwd, reps)
     tb = _timed(torch, stream, bwd, reps)

@@ -224,7 +224,7 @@ KFR_AMD_API size_t kfr_amd_conv_fft_size(KFR_AMD_CONV* conv);
 KFR_AMD_API int kfr_amd_conv_forward(KFR_AMD_CONV* conv, void* y, const void* x, const void* h, size_t h_dist,
                                      size_t batch, int correlate, void* stream);
 KFR_AMD_API int kfr_amd_conv_backward(KFR_AMD_CONV* conv, void* dx, void* dh, const void* g, const void* x,
-                                      const void* h, size_t h_dist, size_t batch, void* stream);
+                                      const void* h, size_t h_dist, size_t batch, int correlate, void* stream);
 KFR_AMD_API void kfr_amd_conv_delete(KFR_AMD_CONV* conv);
 
 /* Uneven batches: `count` complex transforms of lengths sizes[i] and directions inverse[i] (NULL: all

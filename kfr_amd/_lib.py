@@ -80,7 +80,7 @@ def _proto(lib):
     f("kfr_amd_conv_create", vp, C.c_int, c_size, c_size)
     f("kfr_amd_conv_fft_size", c_size, vp)
     f("kfr_amd_conv_forward", C.c_int, vp, vp, vp, vp, c_size, c_size, C.c_int, vp)
-    f("kfr_amd_conv_backward", C.c_int, vp, vp, vp, vp, vp, vp, c_size, c_size, vp)
+    f("kfr_amd_conv_backward", C.c_int, vp, vp, vp, vp, vp, vp, c_size, c_size, C.c_int, vp)
     f("kfr_amd_conv_delete", None, vp)
     f("kfr_amd_describe", C.c_int, C.c_int, C.c_int, c_size, C.c_int, C.c_char_p, c_size)
     f("kfr_amd_dft_permutation", C.c_int, C.c_int, c_size, vp)

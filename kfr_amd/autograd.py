@@ -1,7 +1,8 @@
 """Differentiable batched FFT convolution for PyTorch over the C ABI.
 
 `fft_convolve(x, h)` computes y[b] = x[b] * h[b] (linear convolution, len nx + nh - 1) with the GPU
-real-FFT convolution; its backward pass calls the hand-written adjoint kernels: the signal gradient
+real-FFT convolution, and `fft_correlate(x, h)` the correlation (the convolution with h reversed, as the
+reference's correlate); their backward pass calls the hand-written adjoint kernels: the signal gradient
 is the correlation of the incoming gradient with the taps and the tap gradient is the correlation
 with the signal, both through the conjugate spectrum, with the signal spectrum recomputed rather than
 stored. Supports float32 and float64 CUDA tensors, h of shape (nh,) (shared) or (B, nh).
@@ -31,7 +32,7 @@ def _stream():
 
 class FFTConvolve(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, x: torch.Tensor, h: torch.Tensor):
+    def forward(ctx, x: torch.Tensor, h: torch.Tensor, correlate: bool = False):
         if x.dtype not in (torch.float32, torch.float64) or h.dtype != x.dtype:
             raise TypeError("x and h must both be float32 or both float64")
         if not (x.is_cuda and h.is_cuda):
@@ -45,11 +46,12 @@ class FFTConvolve(torch.autograd.Function):
             raise ValueError("h must have shape (nh,) or (batch, nh)")
         y = torch.empty(B, nx + nh - 1, dtype=x.dtype, device=x.device)
         p = _plan(x.dtype, nx, nh)
-        r = lib().kfr_amd_conv_forward(p, y.data_ptr(), x2.data_ptr(), h2.data_ptr(), 0 if shared else nh, B, 0,
-                                       _stream())
+        r = lib().kfr_amd_conv_forward(p, y.data_ptr(), x2.data_ptr(), h2.data_ptr(), 0 if shared else nh, B,
+                                       int(bool(correlate)), _stream())
         check(r == 0)
         ctx.save_for_backward(x2, h2)
         ctx.shared = shared
+        ctx.correlate = bool(correlate)
         ctx.xshape = x.shape
         ctx.hshape = h.shape
         return y.reshape(x.shape[:-1] + (nx + nh - 1,))
@@ -66,11 +68,17 @@ class FFTConvolve(torch.autograd.Function):
         p = _plan(x2.dtype, nx, nh)
         r = lib().kfr_amd_conv_backward(p, dx.data_ptr() if dx is not None else None,
                                         dh.data_ptr() if dh is not None else None, g2.data_ptr(), x2.data_ptr(),
-                                        h2.data_ptr(), 0 if ctx.shared else nh, B, _stream())
+                                        h2.data_ptr(), 0 if ctx.shared else nh, B, int(ctx.correlate), _stream())
         check(r == 0)
         return (dx.reshape(ctx.xshape) if dx is not None else None,
-                dh.reshape(ctx.hshape) if dh is not None else None)
+                dh.reshape(ctx.hshape) if dh is not None else None, None)
 
 
 def fft_convolve(x: torch.Tensor, h: torch.Tensor) -> torch.Tensor:
-    return FFTConvolve.apply(x, h)
+    return FFTConvolve.apply(x, h, False)
+
+
+def fft_correlate(x: torch.Tensor, h: torch.Tensor) -> torch.Tensor:
+    """y[b][m] = sum_j x[b][m - j] h[b][nh - 1 - j]: the correlation of include/kfr/dft/convolution.hpp:62-67
+    (convolution with h reversed), differentiable."""
+    return FFTConvolve.apply(x, h, True)

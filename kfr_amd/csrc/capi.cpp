@@ -102,6 +102,45 @@ bool is_device_ptr(const void* p)
 // NULL is the HIP default (null) stream, as in HIP and PyTorch (whose default stream handle is 0)
 hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
+// Orders the uses of one plan's device scratch (four-step intermediates, real and DCT half spectra,
+// Bluestein buffers) across streams: work enqueued on a stream other than the one that used the
+// plan last first waits for that stream's last work on this plan. Enqueues from several host
+// threads are serialised by the mutex, which also guards the lazily grown scratch. Inside a stream
+// capture nothing is recorded (the captured stream orders its own work).
+struct stream_order
+{
+    std::mutex mu;
+    hipEvent_t ev    = nullptr;
+    hipStream_t last = nullptr;
+    bool used        = false;
+    stream_order()   = default;
+    stream_order(const stream_order&) = delete;
+    ~stream_order()
+    {
+        if (ev)
+            (void)hipEventDestroy(ev);
+    }
+    template <typename Fn>
+    void run(hipStream_t s, Fn&& fn)
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        hip_check(hipStreamIsCapturing(s, &cs), "stream capture status");
+        const bool capturing = cs != hipStreamCaptureStatusNone;
+        if (!capturing && used && last != s)
+            hip_check(hipStreamWaitEvent(s, ev, 0), "stream ordering");
+        fn();
+        if (!capturing)
+        {
+            if (!ev)
+                hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
+            hip_check(hipEventRecord(ev, s), "stream ordering");
+            last = s;
+            used = true;
+        }
+    }
+};
+
 // Runs fn(dev_out, dev_in) on device buffers: the user's pointers when both are device memory,
 // otherwise one staging buffer (the transforms are safe in place for a single item) with copies.
 struct stager
@@ -115,6 +154,9 @@ struct stager
         hipStream_t s = library_stream();
         if (is_device_ptr(in) && is_device_ptr(out))
         {
+            // the reference's execute is synchronous: device inputs may still be in production on any
+            // stream (the library's stream does not wait for the null stream), so wait for the device
+            hip_check(hipDeviceSynchronize(), "execute");
             fn(out, in, s);
             hip_check(hipStreamSynchronize(s), "execute");
             return;
@@ -157,16 +199,22 @@ struct dft_obj
         return t;
     }
     std::string schedule() const { return g ? g->schedule().text() : md->schedule(); }
+    stream_order ord;
     void execute(void* out, const void* in, long long batch, bool inverse, hipStream_t s)
     {
-        if (g)
-        {
-            g->execute(out, in, batch, inverse, s);
-            return;
-        }
-        const size_t bytes = md->size() * 2 * sizeof(T);
-        for (long long b = 0; b < batch; ++b)
-            md->execute(static_cast<char*>(out) + b * bytes, static_cast<const char*>(in) + b * bytes, inverse, s);
+        ord.run(s,
+                [&]
+                {
+                    if (g)
+                    {
+                        g->execute(out, in, batch, inverse, s);
+                        return;
+                    }
+                    const size_t bytes = md->size() * 2 * sizeof(T);
+                    for (long long b = 0; b < batch; ++b)
+                        md->execute(static_cast<char*>(out) + b * bytes, static_cast<const char*>(in) + b * bytes,
+                                    inverse, s);
+                });
     }
 };
 
@@ -215,27 +263,36 @@ struct real_obj
         return t + extra_temp;
     }
     std::string schedule() const { return g ? g->schedule().text() : md->schedule(); }
+    stream_order ord;
     void forward(void* out, const void* in, long long batch, hipStream_t s)
     {
-        if (g)
-        {
-            g->forward(out, in, batch, s);
-            return;
-        }
-        for (long long b = 0; b < batch; ++b)
-            md->forward(static_cast<char*>(out) + b * complex_size() * 2 * sizeof(T),
-                        static_cast<const char*>(in) + b * size() * sizeof(T), s);
+        ord.run(s,
+                [&]
+                {
+                    if (g)
+                    {
+                        g->forward(out, in, batch, s);
+                        return;
+                    }
+                    for (long long b = 0; b < batch; ++b)
+                        md->forward(static_cast<char*>(out) + b * complex_size() * 2 * sizeof(T),
+                                    static_cast<const char*>(in) + b * size() * sizeof(T), s);
+                });
     }
     void inverse(void* out, const void* in, long long batch, hipStream_t s)
     {
-        if (g)
-        {
-            g->inverse(out, in, batch, s);
-            return;
-        }
-        for (long long b = 0; b < batch; ++b)
-            md->inverse(static_cast<char*>(out) + b * size() * sizeof(T),
-                        static_cast<const char*>(in) + b * complex_size() * 2 * sizeof(T), s);
+        ord.run(s,
+                [&]
+                {
+                    if (g)
+                    {
+                        g->inverse(out, in, batch, s);
+                        return;
+                    }
+                    for (long long b = 0; b < batch; ++b)
+                        md->inverse(static_cast<char*>(out) + b * size() * sizeof(T),
+                                    static_cast<const char*>(in) + b * complex_size() * 2 * sizeof(T), s);
+                });
     }
 };
 template <typename T>
@@ -244,7 +301,12 @@ struct dct_obj
     plan_desc desc;
     gpu_dct<T> g;
     stager st;
+    stream_order ord;
     explicit dct_obj(size_t n) : desc(describe_dct(sizeof(T) == 4 ? dtype_t::f32 : dtype_t::f64, n)), g(n) {}
+    void execute(void* out, const void* in, long long batch, bool inverse, hipStream_t s)
+    {
+        ord.run(s, [&] { g.execute(out, in, batch, inverse, s); });
+    }
 };
 template <typename T>
 struct filter_obj
@@ -305,7 +367,7 @@ void dct_exec(void* plan, void* out, const void* in, bool inverse)
 {
     auto* o        = dct_cast<T>(plan);
     const size_t b = o->g.size() * sizeof(T);
-    o->st.run(out, b, in, b, [&](void* dout, const void* din, hipStream_t s) { o->g.execute(dout, din, 1, inverse, s); });
+    o->st.run(out, b, in, b, [&](void* dout, const void* din, hipStream_t s) { o->execute(dout, din, 1, inverse, s); });
 }
 
 template <typename T>
@@ -384,7 +446,7 @@ struct conv_obj
     int dtype;
     std::unique_ptr<gpu_conv<float>> f;
     std::unique_ptr<gpu_conv<double>> d;
-    std::mutex mu;
+    stream_order ord;
 };
 
 struct batch_obj
@@ -393,14 +455,19 @@ struct batch_obj
     std::unique_ptr<gpu_batch<double>> d;
     dev_mem buf; // staging for host pointers
     std::mutex mu;
+    stream_order ord;
     size_t total() const { return f ? f->total() : d->total(); }
     size_t elem() const { return f ? sizeof(float) * 2 : sizeof(double) * 2; }
     void run(void* out, const void* in, hipStream_t s)
     {
-        if (f)
-            f->execute(out, in, s);
-        else
-            d->execute(out, in, s);
+        ord.run(s,
+                [&]
+                {
+                    if (f)
+                        f->execute(out, in, s);
+                    else
+                        d->execute(out, in, s);
+                });
     }
 };
 
@@ -410,6 +477,11 @@ struct half_obj
     size_t n = 0;
     dev_mem buf; // staging for host pointers
     std::mutex mu;
+    stream_order ord;
+    void run(bool inverse, const void* in, void* out, long long batch, hipStream_t s)
+    {
+        ord.run(s, [&] { f->run(inverse, in, out, batch, s); });
+    }
 };
 
 int copy_text(const std::string& s, char* buf, size_t buflen)
@@ -849,7 +921,7 @@ extern "C"
         return try_fn(
             [&]
             {
-                dct_cast<float>(plan)->g.execute(out, in, (long long)batch, inverse != 0, as_stream(stream));
+                dct_cast<float>(plan)->execute(out, in, (long long)batch, inverse != 0, as_stream(stream));
                 return 0;
             },
             -1);
@@ -860,7 +932,7 @@ extern "C"
         return try_fn(
             [&]
             {
-                dct_cast<double>(plan)->g.execute(out, in, (long long)batch, inverse != 0, as_stream(stream));
+                dct_cast<double>(plan)->execute(out, in, (long long)batch, inverse != 0, as_stream(stream));
                 return 0;
             },
             -1);
@@ -939,29 +1011,39 @@ extern "C"
             [&]
             {
                 check_plan(conv);
-                auto* c = reinterpret_cast<conv_obj*>(conv);
-                std::lock_guard<std::mutex> lock(c->mu);
-                if (c->f)
-                    c->f->forward(y, x, h, (long long)h_dist, (long long)batch, correlate != 0, as_stream(stream));
-                else
-                    c->d->forward(y, x, h, (long long)h_dist, (long long)batch, correlate != 0, as_stream(stream));
+                auto* c       = reinterpret_cast<conv_obj*>(conv);
+                hipStream_t s = as_stream(stream);
+                c->ord.run(s,
+                           [&]
+                           {
+                               if (c->f)
+                                   c->f->forward(y, x, h, (long long)h_dist, (long long)batch, correlate != 0, s);
+                               else
+                                   c->d->forward(y, x, h, (long long)h_dist, (long long)batch, correlate != 0, s);
+                           });
                 return 0;
             },
             -1);
     }
     int kfr_amd_conv_backward(KFR_AMD_CONV* conv, void* dx, void* dh, const void* g, const void* x, const void* h,
-                              size_t h_dist, size_t batch, void* stream)
+                              size_t h_dist, size_t batch, int correlate, void* stream)
     {
         return try_fn(
             [&]
             {
                 check_plan(conv);
-                auto* c = reinterpret_cast<conv_obj*>(conv);
-                std::lock_guard<std::mutex> lock(c->mu);
-                if (c->f)
-                    c->f->backward(dx, dh, g, x, h, (long long)h_dist, (long long)batch, as_stream(stream));
-                else
-                    c->d->backward(dx, dh, g, x, h, (long long)h_dist, (long long)batch, as_stream(stream));
+                auto* c       = reinterpret_cast<conv_obj*>(conv);
+                hipStream_t s = as_stream(stream);
+                c->ord.run(s,
+                           [&]
+                           {
+                               if (c->f)
+                                   c->f->backward(dx, dh, g, x, h, (long long)h_dist, (long long)batch,
+                                                  correlate != 0, s);
+                               else
+                                   c->d->backward(dx, dh, g, x, h, (long long)h_dist, (long long)batch,
+                                                  correlate != 0, s);
+                           });
                 return 0;
             },
             -1);
@@ -1156,14 +1238,14 @@ extern "C"
                 std::lock_guard<std::mutex> lock(h->mu);
                 if (is_device_ptr(in) && is_device_ptr(out))
                 {
-                    h->f->run(inverse != 0, in, out, (long long)batch, as_stream(stream));
+                    h->run(inverse != 0, in, out, (long long)batch, as_stream(stream));
                     return 0;
                 }
                 const size_t bytes = batch * h->n * 4;
                 hipStream_t s      = library_stream();
                 h->buf.ensure(bytes);
                 hip_check(hipMemcpyAsync(h->buf.ptr, in, bytes, hipMemcpyDefault, s), "input copy");
-                h->f->run(inverse != 0, h->buf.ptr, h->buf.ptr, (long long)batch, s);
+                h->run(inverse != 0, h->buf.ptr, h->buf.ptr, (long long)batch, s);
                 hip_check(hipMemcpyAsync(out, h->buf.ptr, bytes, hipMemcpyDefault, s), "output copy");
                 hip_check(hipStreamSynchronize(s), "execute");
                 return 0;

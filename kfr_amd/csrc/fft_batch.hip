@@ -197,6 +197,7 @@ template <typename T>
 len_desc make_desc(const host_len<T>& h, int tpb)
 {
     len_desc d;
+    check_pass_radices<T>(h.S, true);
     d.G         = make_geom(h.S, tpb, 0, 1, (int)h.m);
     d.tw        = h.tw;
     d.chirp     = h.chirp;

@@ -728,6 +728,21 @@ inline bool is_mixed(const stockham& s)
     return false;
 }
 
+// fft_passes runs a pass only if its build has the butterfly: radix 16 where a thread holds 16 values,
+// odd radices and direct prime passes in the mixed-radix build. A schedule with any other pass would
+// leave it out silently, so launches check on the host first.
+template <typename T>
+inline void check_pass_radices(const stockham& s, bool mixed_build)
+{
+    for (int r : s.radix)
+    {
+        const bool ok = r == 2 || r == 4 || r == 8 || (r == 16 && pmax_of<T> >= 16) ||
+                        (mixed_build && (r & (r - 1)) != 0 && r > 2);
+        if (!ok)
+            throw std::logic_error("no butterfly for a radix-" + std::to_string(r) + " pass in this kernel build");
+    }
+}
+
 // launches the power-of-two or the mixed-radix build of a kernel role
 template <typename T, bool INV, int ROLE>
 void launch_fft(bool mixed, const lds_geom& G, const cx<T>* tw, long long batch, const xload<T>& ld,
@@ -801,12 +816,15 @@ struct core_fft
         if (fits_lds(n))
         {
             make_stockham((long long)n, S, pmax_of<T>);
+            check_pass_radices<T>(S, is_mixed(S));
             tpb = pick_tpb(S);
             tw.reset((size_t)std::max<long long>(1, S.tw_count) * sizeof(cx<T>));
             gen_stockham_twiddles(S, tw.as<cx<T>>(), s);
             return;
         }
         plan_fourstep();
+        check_pass_radices<T>(SA, is_mixed(SA));
+        check_pass_radices<T>(SB, is_mixed(SB));
         fourstep            = true;
         lbits               = (ilog2_ceil(n) + 1) / 2;
         const long long nTB = 1ll << lbits;

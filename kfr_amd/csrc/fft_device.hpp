@@ -315,8 +315,13 @@ constexpr int max_passes = 20;
 // only, with up to 1024 threads per block (128 VGPRs each), and mixed radices (odd butterflies and
 // direct prime passes need more temporaries), with up to 512 threads (256 VGPRs each). Both fit
 // without spilling to scratch.
+// The f64 value is a build option so that the CPU emulation of these kernels (tests/cpu_emu) can check
+// the 16-points-per-thread geometry for f64 too; the library is built with 8.
+#ifndef KFR_AMD_PMAX_F64
+#define KFR_AMD_PMAX_F64 8
+#endif
 template <typename T>
-inline constexpr int pmax_of = sizeof(T) == 4 ? 16 : 8;
+inline constexpr int pmax_of = sizeof(T) == 4 ? 16 : KFR_AMD_PMAX_F64;
 constexpr int max_block_pow2  = 1024;
 constexpr int max_block_mixed = 512;
 __host__ __device__ constexpr int max_block_of(bool mixed) { return mixed ? max_block_mixed : max_block_pow2; }
@@ -499,8 +504,8 @@ __device__ __forceinline__ void fft_passes(const lds_geom& G, const cx<T>* tw, c
         case 8:
             radix_pass<T, 8, INV, WIDE>(G, P, tw, c);
             break;
-        case 16:
-            if constexpr (sizeof(T) == 4)
+        case 16: // only where a thread holds 16 values (check_pass_radices rejects it elsewhere)
+            if constexpr (pmax_of<T> >= 16)
                 radix_pass<T, 16, INV, WIDE>(G, P, tw, c);
             break;
         default:

@@ -236,6 +236,7 @@ struct gpu_dft<T>::impl
         {
             algo = BS_FUSED;
             make_stockham((long long)M, bsS, pmax_of<T>);
+            check_pass_radices<T>(bsS, false);
             bs_tpb = core_fft<T>::pick_tpb(bsS);
             bs_tw.reset((size_t)std::max<long long>(1, bsS.tw_count) * sizeof(cx<T>));
             gen_stockham_twiddles(bsS, bs_tw.as<cx<T>>(), s);
@@ -972,6 +973,22 @@ void gpu_conv<T>::forward(void* y, const void* x, const void* h, long long h_dis
 template <typename T>
 __global__ void k_zero(T* p, long long count);
 
+// a[b*n + j] <-> a[b*n + n-1-j] for every item b < items: thread i swaps pair j = i mod (n/2)
+template <typename T>
+__global__ void k_reverse_items(T* a, long long n, long long items)
+{
+    const long long half = n / 2;
+    const long long i    = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (half == 0 || i >= items * half)
+        return;
+    const long long b = i / half;
+    const long long j = i - b * half;
+    T* p              = a + b * n;
+    const T t         = p[j];
+    p[j]              = p[n - 1 - j];
+    p[n - 1 - j]      = t;
+}
+
 template <typename T>
 __global__ void k_sum_batch(T* out, const T* in, long long n, long long batch)
 {
@@ -986,7 +1003,7 @@ __global__ void k_sum_batch(T* out, const T* in, long long n, long long batch)
 
 template <typename T>
 void gpu_conv<T>::backward(void* dx, void* dh, const void* g, const void* x, const void* h, long long h_dist,
-                           long long batch, hipStream_t s)
+                           long long batch, bool correlate, hipStream_t s)
 {
     impl& q = *p;
     if (batch <= 0)
@@ -1000,10 +1017,21 @@ void gpu_conv<T>::backward(void* dx, void* dh, const void* g, const void* x, con
         }
         return;
     }
+    // the correlation's filter is h reversed: its gradient is the convolution's, reversed
+    const long long dh_items = h_dist == 0 ? 1 : batch;
+    auto reverse_dh          = [&]
+    {
+        if (!dh || !correlate)
+            return;
+        hipLaunchKernelGGL(k_reverse_items<T>, dim3(nblocks(dh_items * (long long)(q.nh / 2), 256) + 1), dim3(256), 0,
+                           s, static_cast<T*>(dh), (long long)q.nh, dh_items);
+        hip_check(hipGetLastError(), "gradient reversal");
+    };
     if (q.os)
     {
         q.os->backward(static_cast<T*>(dx), static_cast<T*>(dh), static_cast<const T*>(g), static_cast<const T*>(x),
-                       static_cast<const T*>(h), h_dist, batch, s);
+                       static_cast<const T*>(h), h_dist, batch, correlate, s);
+        reverse_dh();
         return;
     }
     const long long cs = q.cs();
@@ -1015,8 +1043,9 @@ void gpu_conv<T>::backward(void* dx, void* dh, const void* g, const void* x, con
                     s);
     if (dx)
     {
-        // dx = g correlated with h: IFFT(G conj(H)), first nx samples
-        q.r->forward_op(load_real_pad<T>(h, h_dist == 0 ? 0 : (long long)q.nh, (long long)q.nh, false),
+        // dx = g correlated with the filter f (h, or h reversed for the correlation): IFFT(G conj(F)), first
+        // nx samples
+        q.r->forward_op(load_real_pad<T>(h, h_dist == 0 ? 0 : (long long)q.nh, (long long)q.nh, correlate),
                         q.Hs.template as<cx<T>>(), cs, hb, s);
         q.r->inverse_op(spec_product<T>(q.G.ptr, cs, q.Hs.template as<cx<T>>(), h_dist == 0 ? 0 : cs, scale, true),
                         store_real_trunc<T>(dx, (long long)q.nx, (long long)q.nx, T(1)), batch, s);
@@ -1040,6 +1069,7 @@ void gpu_conv<T>::backward(void* dx, void* dh, const void* g, const void* x, con
                                static_cast<T*>(dh), q.tmp.template as<T>(), (long long)q.nh, batch);
             hip_check(hipGetLastError(), "gradient reduction");
         }
+        reverse_dh();
     }
 }
 

@@ -123,10 +123,12 @@ public:
     // y[b] = x[b] conv h[b] (h_dist = 0 shares one filter); correlate: h reversed
     void forward(void* y, const void* x, const void* h, long long h_dist, long long batch, bool correlate,
                  hipStream_t s);
-    // adjoint of forward: dx = g corr h (first nx), dh = g corr x (first nh); the spectrum of x is
-    // recomputed, not stored. dh is accumulated over the batch when h_dist == 0.
+    // adjoint of forward (with the same correlate flag): for convolution dx = g corr h (first nx) and
+    // dh = g corr x (first nh); for correlation the filter is h reversed, so dx = g conv h and dh is
+    // g corr x reversed. The spectrum of x is recomputed, not stored. dh is accumulated over the batch
+    // when h_dist == 0.
     void backward(void* dx, void* dh, const void* g, const void* x, const void* h, long long h_dist,
-                  long long batch, hipStream_t s);
+                  long long batch, bool correlate, hipStream_t s);
     struct impl;
     std::unique_ptr<impl> p;
 };

@@ -1563,7 +1563,7 @@ void os_conv<T>::forward(T* y, const T* x, const T* h, long long h_dist, long lo
 
 template <typename T>
 void os_conv<T>::backward(T* dx, T* dh, const T* g, const T* x, const T* h, long long h_dist, long long batch,
-                          hipStream_t s)
+                          bool correlate, hipStream_t s)
 {
     if (batch <= 0)
         return;
@@ -1572,9 +1572,10 @@ void os_conv<T>::backward(T* dx, T* dh, const T* g, const T* x, const T* h, long
     const long long N  = 1ll << q.L;
     if (dx)
     {
-        // dx[n] = sum_j g[n + j] h[j] = (g * reversed h)[n + nh - 1]
+        // dx[n] = sum_j g[n + j] f[j] = (g * reversed f)[n + nh - 1], f = h (convolution) or h reversed
+        // (correlation)
         const long long nfil = h_dist == 0 ? 1 : batch;
-        const cx<T>* H       = filter_spectra<T>(q, h, h_dist, nfil, true, s);
+        const cx<T>* H       = filter_spectra<T>(q, h, h_dist, nfil, !correlate, s);
         run_os<T>(q, dx, g, ny, ny, (long long)q.nh - 1, (long long)q.nx, (long long)q.nx, H,
                   h_dist == 0 ? 0 : N, batch, s);
     }

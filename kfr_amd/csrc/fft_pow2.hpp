@@ -54,10 +54,11 @@ public:
     ~os_conv();
     // y[b] = x[b] * h[b] (h_dist = 0: one shared filter); correlate: h reversed. |y| = nx + nh - 1
     void forward(T* y, const T* x, const T* h, long long h_dist, long long batch, bool correlate, hipStream_t s);
-    // dx = g correlated with h (first nx values), dh = g correlated with x (first nh lags; summed over
-    // the batch when h_dist = 0); either may be null
+    // adjoint of forward: dx = g correlated with the filter (h, or h reversed when correlate), and the
+    // filter gradient g correlated with x (first nh lags; summed over the batch when h_dist = 0) into dh,
+    // reversed when correlate is set by the caller (gpu_conv); either output may be null
     void backward(T* dx, T* dh, const T* g, const T* x, const T* h, long long h_dist, long long batch,
-                  hipStream_t s);
+                  bool correlate, hipStream_t s);
     std::string describe() const;
 
     struct impl;

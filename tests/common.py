@@ -12,8 +12,18 @@ EPS = {np.dtype(np.complex64): float(np.finfo(np.float32).eps), np.dtype(np.comp
 
 
 def tol_kfr(n: int, dtype) -> float:
-    """Spectra parity bound: relative RMS error 4*log2(N)*eps of the stated dtype."""
+    """Spectra parity bound against the reference engine's own output: relative RMS error
+    4*log2(N)*eps of the stated dtype."""
     return 4.0 * max(1.0, math.log2(max(n, 2))) * EPS[np.dtype(dtype)]
+
+
+def bound_fp64(n: int, dtype) -> float:
+    """Bound on the relative RMS error against an FP64 (or better) DFT: 1e-6 for single precision at
+    every length (the accuracy target of the single-precision path), 4*log2(N)*eps but at least 1e-14
+    for double precision."""
+    if np.dtype(dtype) in (np.dtype(np.complex64), np.dtype(np.float32)):
+        return 1e-6
+    return max(1e-14, tol_kfr(n, dtype))
 
 
 def fp64_dft(x: np.ndarray, inverse: bool = False) -> np.ndarray:

@@ -1,0 +1,95 @@
+"""The engine's HIP kernels run on the CPU through an emulation of the HIP runtime (tests/cpu_emu):
+every workgroup as host threads meeting at its barriers, its LDS as one heap block of exactly the
+launch's size, device buffers as exact host allocations. Built with AddressSanitizer, any LDS or
+global access outside its buffer aborts the run; the spectra are checked against numpy in FP64.
+
+Cases: the generic four-step over factors 3, 5, 7 with workgroups that straddle batch items
+(f32 at 16 points per thread), the power-of-two four-step, the mixed-radix menu lengths, Bluestein, a
+real transform and a DCT; and the f64 generic kernels built at 16 points per thread (the geometry an
+earlier build ran on the GPU, where lengths with radix-5 passes came out wrong) on those lengths.
+Set KFR_AMD_EMU_TSAN=1 to also run a ThreadSanitizer build (a missing barrier shows as a race)."""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+HERE = Path(__file__).resolve().parent
+sys.path.insert(0, str(HERE / "cpu_emu"))
+import build as emu_build  # noqa: E402
+
+import oracle  # noqa: E402
+from tests.common import fp64_dct, fp64_rdft  # noqa: E402
+
+pytestmark = pytest.mark.skipif(not Path(emu_build.CXX).exists(), reason="ROCm clang++ not found")
+
+
+def _run(variant, tmp_path, args, x, out_dtype):
+    exe = emu_build.build(variant)
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    np.ascontiguousarray(x).tofile(fin)
+    r = subprocess.run([str(exe)] + [str(a) for a in args] + [str(fin), str(fout)], capture_output=True, text=True,
+                       timeout=900)
+    assert r.returncode == 0, r.stderr[-4000:]
+    return np.fromfile(fout, dtype=out_dtype)
+
+
+def _dft(variant, tmp_path, dtype, n, batch, inverse, seed):
+    t = "f32" if dtype == np.complex64 else "f64"
+    x = oracle.complex_uniform(seed, (batch, n), dtype)
+    y = _run(variant, tmp_path, ["dft", t, n, batch, int(inverse)], x, dtype).reshape(batch, n)
+    xx = x.astype(np.complex128)
+    ref = np.fft.ifft(xx, axis=1) * n if inverse else np.fft.fft(xx, axis=1)
+    return max(oracle.rel_rms(y[b], ref[b]) for b in range(batch))
+
+
+# (n, batch): 59049 = 3^10 and 30240 = 2^5 3^3 5 7 run the generic four-step with odd column and row
+# counts; 13440 = 2^7 3 5 7 and 62500 = 2^2 5^6 are the lengths that failed at 16 f64 points per
+# thread; 65536 the power-of-two four-step; 3840 a menu length; 10007 chained Bluestein (M = 2^15); 97 a direct
+# prime pass
+F32_CASES = [(59049, 3), (30240, 3), (13440, 2), (1 << 16, 2), (3840, 3), (10007, 1), (97, 5)]
+F64_CASES = [(13440, 3), (15360, 2), (30240, 3), (62500, 1)]
+
+
+@pytest.mark.parametrize("n,batch", F32_CASES)
+def test_emulated_f32_lengths(n, batch, tmp_path):
+    for inverse in (False, True):
+        e = _dft("asan", tmp_path, np.complex64, n, batch, inverse, 40 + n % 97)
+        print(f"emulated f32 n={n} batch={batch} inv={inverse}: rel_rms_fp64={e:.3e}", flush=True)
+        assert e <= 1e-6
+
+
+@pytest.mark.parametrize("variant", ["asan", "asan_pm16"])
+@pytest.mark.parametrize("n,batch", F64_CASES)
+def test_emulated_f64_lengths(variant, n, batch, tmp_path):
+    for inverse in (False, True):
+        e = _dft(variant, tmp_path, np.complex128, n, batch, inverse, 50 + n % 89)
+        print(f"emulated f64 ({variant}) n={n} batch={batch} inv={inverse}: rel_rms_fp64={e:.3e}", flush=True)
+        assert e <= 1e-14
+
+
+def test_emulated_real_and_dct(tmp_path):
+    n, batch = 1 << 14, 3
+    x = oracle.gaussian(71, batch * n, np.float32).reshape(batch, n)
+    for fmt in (0, 1):
+        X = _run("asan", tmp_path, ["real", "f32", n, fmt, batch, 0], x, np.complex64).reshape(batch, -1)
+        assert oracle.rel_rms(X, fp64_rdft(x.astype(np.float64), fmt)) <= 1e-6
+        y = _run("asan", tmp_path, ["real", "f32", n, fmt, batch, 1], X, np.float32).reshape(batch, n)
+        assert oracle.rel_rms(y / n, x) <= 1e-6
+    m = 4096
+    d = oracle.gaussian(72, 2 * m, np.float32).reshape(2, m)
+    for inverse in (False, True):
+        y = _run("asan", tmp_path, ["dct", "f32", m, 2, int(inverse)], d, np.float32).reshape(2, m)
+        assert oracle.rel_rms(y, fp64_dct(d.astype(np.float64), inverse)) <= 2e-6
+
+
+@pytest.mark.skipif(os.environ.get("KFR_AMD_EMU_TSAN") != "1", reason="set KFR_AMD_EMU_TSAN=1")
+@pytest.mark.parametrize("dtype,n,batch", [(np.complex64, 30240, 3), (np.complex64, 1 << 15, 1),
+                                           (np.complex128, 13440, 2)])
+def test_emulated_no_races(dtype, n, batch, tmp_path):
+    e = _dft("tsan", tmp_path, dtype, n, batch, False, 60)
+    assert e <= (1e-6 if dtype == np.complex64 else 1e-14)

@@ -9,13 +9,13 @@ import pytest
 
 import kfr_amd
 import oracle
-from tests.common import fp64_dft, tol_kfr, uneven_lengths
+from tests.common import bound_fp64, fp64_dft, uneven_lengths
 
 pytestmark = pytest.mark.gpu
 
 
 def _bound(n, dtype):
-    return max(1e-6 if dtype == np.complex64 else 1e-14, tol_kfr(n, dtype))
+    return bound_fp64(n, dtype)
 
 
 def _ref(x, inverse, dtype):

@@ -6,7 +6,7 @@ import pytest
 
 import kfr_amd
 import oracle
-from tests.common import fp64_dft, tol_kfr
+from tests.common import bound_fp64, fp64_dft, tol_kfr
 
 pytestmark = pytest.mark.gpu
 
@@ -23,8 +23,7 @@ def _check(n, dtype, inverse, seed):
     plan.execute(out, x, inverse=inverse)
     ref = fp64_dft(x.astype(np.complex128), inverse)
     err = oracle.rel_rms(out, ref)
-    bound = 1e-6 if dtype == np.complex64 else 1e-14
-    bound = max(bound, tol_kfr(n, dtype))
+    bound = bound_fp64(n, dtype)
     print(f"n={n} {np.dtype(dtype).name} inv={inverse} rel_rms_fp64={err:.3e} sched={kfr_amd.schedule('f32' if dtype == np.complex64 else 'f64', n)}",
           flush=True)
     assert err <= bound, (n, dtype, inverse, err)
@@ -85,12 +84,12 @@ def test_dft_batch_device(n, batch, dtype):
         y = yd.cpu().numpy().view(dtype).reshape(batch, n)
         ref = fp64_dft(x.astype(np.complex128), inverse)
         worst = max(oracle.rel_rms(y[b], ref[b]) for b in range(batch))
-        assert worst <= max(1e-6 if dtype == np.complex64 else 1e-14, tol_kfr(n, dtype)), (n, batch, inverse, worst)
+        assert worst <= bound_fp64(n, dtype), (n, batch, inverse, worst)
     # in place
     plan.execute_batch(xd, xd, batch, False)
     torch.cuda.synchronize()
     y = xd.cpu().numpy().view(dtype).reshape(batch, n)
-    assert oracle.rel_rms(y, fp64_dft(x.astype(np.complex128))) <= max(1e-6 if dtype == np.complex64 else 1e-14, tol_kfr(n, dtype))
+    assert oracle.rel_rms(y, fp64_dft(x.astype(np.complex128))) <= bound_fp64(n, dtype)
 
 
 def test_capi_round_trip_known_answer():
@@ -143,7 +142,7 @@ def test_generic_four_step_batch(n, dtype):
     plan = kfr_amd.dft_plan(n, dtype)
     sched = kfr_amd.schedule("f32" if dtype == np.complex64 else "f64", n)
     assert "four-step" in sched, sched
-    bound = max(1e-6 if dtype == np.complex64 else 1e-14, tol_kfr(n, dtype))
+    bound = bound_fp64(n, dtype)
     for inverse in (False, True):
         ref = fp64_dft(x.astype(np.complex128), inverse)
         plan.execute_batch(yd, xd, batch, inverse)
@@ -168,7 +167,7 @@ def test_bluestein_chain_batch(n, dtype):
     plan = kfr_amd.dft_plan(n, dtype)
     sched = kfr_amd.schedule("f32" if dtype == np.complex64 else "f64", n)
     assert "bluestein " in sched and "m=262144" in sched, sched
-    bound = max(1e-6 if dtype == np.complex64 else 1e-14, tol_kfr(n, dtype))
+    bound = bound_fp64(n, dtype)
     for inverse in (False, True):
         ref = fp64_dft(x.astype(np.complex128), inverse)
         plan.execute_batch(yd, xd, batch, inverse)

@@ -9,7 +9,7 @@ import pytest
 
 import kfr_amd
 import oracle
-from tests.common import tol_kfr
+from tests.common import bound_fp64
 
 pytestmark = pytest.mark.gpu
 
@@ -20,8 +20,7 @@ SHAPES = [(6, 8, 10), (32, 64), (3, 1, 5, 4), (101, 16), (1024, 128), (64, 64, 6
 
 
 def _bound(shape, dtype):
-    return max(1e-6 if dtype in (np.complex64, np.float32) else 1e-14,
-               tol_kfr(int(np.prod(shape)), dtype))
+    return bound_fp64(int(np.prod(shape)), dtype)
 
 
 @pytest.mark.parametrize("shape", SHAPES)

@@ -8,7 +8,7 @@ import pytest
 
 import kfr_amd
 import oracle
-from tests.common import fp64_dft, tol_kfr
+from tests.common import bound_fp64, fp64_dft
 
 pytestmark = pytest.mark.gpu
 
@@ -25,7 +25,7 @@ def _check(n, batch, seed):
     p = kfr_amd.dft_plan(n, np.complex64)
     sched = kfr_amd.schedule("f32", n)
     assert "mixed" in sched, sched
-    bound = max(1e-6, tol_kfr(n, np.complex64))
+    bound = bound_fp64(n, np.complex64)
     for inverse in (False, True):
         p.execute_batch(yd, xd, batch, inverse)
         torch.cuda.synchronize()
@@ -66,4 +66,4 @@ def test_real_30720000():
         ref = np.fft.rfft(x[b].astype(np.float64))
         e = oracle.rel_rms(Y[b], ref)
         print(f"r2c 30720000 rel_rms={e:.3e}", flush=True)
-        assert e <= max(1e-6, tol_kfr(n, np.float32))
+        assert e <= bound_fp64(n, np.float32)

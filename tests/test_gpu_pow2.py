@@ -8,7 +8,7 @@ import pytest
 
 import kfr_amd
 import oracle
-from tests.common import fp64_dft, tol_kfr
+from tests.common import bound_fp64, fp64_dft
 
 pytestmark = pytest.mark.gpu
 
@@ -30,7 +30,7 @@ def test_pow2_batch(L, dtype):
     yd = torch.empty_like(xd)
     plan = kfr_amd.dft_plan(n, dtype)
     sched = kfr_amd.schedule("f32" if dtype == np.complex64 else "f64", n)
-    bound = max(1e-6 if dtype == np.complex64 else 1e-14, tol_kfr(n, dtype))
+    bound = bound_fp64(n, dtype)
     ref = {inv: fp64_dft(x.astype(np.complex128), inv) for inv in (False, True)}
     for inverse in (False, True):
         plan.execute_batch(yd, xd, batch, inverse)
@@ -86,7 +86,7 @@ def _check_plan(dtype, L, batch, seed):
     xd = torch.from_numpy(x.view(rdt).copy()).cuda()
     yd = torch.empty_like(xd)
     plan = kfr_amd.dft_plan(n, dtype)
-    bound = max(1e-6 if dtype == np.complex64 else 1e-14, tol_kfr(n, dtype))
+    bound = bound_fp64(n, dtype)
     worst = 0.0
     for inverse in (False, True):
         ref = fp64_dft(x.astype(np.complex128), inverse)

@@ -9,7 +9,7 @@ import pytest
 
 import kfr_amd
 import oracle
-from tests.common import fp64_dct, fp64_rdft, tol_kfr
+from tests.common import bound_fp64, fp64_dct, fp64_rdft, tol_kfr
 
 pytestmark = pytest.mark.gpu
 GOLD = Path(__file__).resolve().parent / "golden"
@@ -28,7 +28,7 @@ def test_real_dft(n, fmt, dt):
     p.execute(X, x)
     ref = fp64_rdft(x.astype(np.float64), fmt)
     e = oracle.rel_rms(X, ref)
-    bound = max(1e-6 if dt == np.float32 else 1e-14, tol_kfr(n, dt))
+    bound = bound_fp64(n, dt)
     print(f"real n={n} fmt={fmt} {np.dtype(dt).name} fwd rel_rms={e:.3e}", flush=True)
     assert e <= bound
     y = np.empty(n, dtype=dt)
@@ -202,6 +202,46 @@ def test_batched_conv_forward_and_adjoint_fd_fp64():
     assert worst < 1e-8
 
 
+@pytest.mark.parametrize("nx,nh", [(4096, 255), (300, 200)])
+def test_batched_correlate_adjoint_fd_fp64(nx, nh):
+    """the correlation (convolution with h reversed) and its adjoint, on the overlap-save (4096 x 255)
+    and the full-length (300 x 200) paths: forward against numpy, gradients against FP64 central
+    differences on 64 coordinates (32 of the signal, 32 of the taps)"""
+    torch = pytest.importorskip("torch")
+    from kfr_amd.autograd import fft_correlate
+    B = 2
+    x = torch.from_numpy(oracle.gaussian(21, B * nx, np.float64).reshape(B, nx)).cuda().requires_grad_(True)
+    h = torch.from_numpy(oracle.gaussian(22, B * nh, np.float64).reshape(B, nh)).cuda().requires_grad_(True)
+    y = fft_correlate(x, h)
+    yr = np.stack([np.convolve(x[b].detach().cpu().numpy(), h[b].detach().cpu().numpy()[::-1]) for b in range(B)])
+    assert oracle.rel_rms(y.detach().cpu().numpy(), yr) < 1e-13
+    w = torch.from_numpy(oracle.gaussian(23, B * (nx + nh - 1), np.float64).reshape(B, -1)).cuda()
+    (y * w).sum().backward()
+    rng = np.random.default_rng(8)
+
+    def loss(xv, hv):
+        return float((fft_correlate(xv, hv) * w).sum())
+
+    eps = 1e-3
+    worst = 0.0
+    for k in range(64):
+        on_x = k < 32
+        b = int(rng.integers(B))
+        i = int(rng.integers(nx if on_x else nh))
+        v = (x if on_x else h).detach()
+        vp, vm = v.clone(), v.clone()
+        vp[b, i] += eps
+        vm[b, i] -= eps
+        if on_x:
+            fd = (loss(vp, h.detach()) - loss(vm, h.detach())) / (2 * eps)
+            worst = max(worst, abs(fd - float(x.grad[b, i])))
+        else:
+            fd = (loss(x.detach(), vp) - loss(x.detach(), vm)) / (2 * eps)
+            worst = max(worst, abs(fd - float(h.grad[b, i])))
+    print(f"correlate adjoint nx={nx} nh={nh} vs central differences: max abs diff {worst:.3e}", flush=True)
+    assert worst < 1e-8
+
+
 def test_batched_conv_adjoint_matches_torch_fft_reference():
     torch = pytest.importorskip("torch")
     from kfr_amd.autograd import fft_convolve
@@ -309,7 +349,7 @@ def test_conv_empty_batch():
         x = torch.zeros(8, device="cuda")
         h = torch.zeros(nh, device="cuda")
         assert L.kfr_amd_conv_forward(c, y.data_ptr(), x.data_ptr(), h.data_ptr(), nh, 0, 0, None) == 0
-        assert L.kfr_amd_conv_backward(c, None, dh.data_ptr(), x.data_ptr(), x.data_ptr(), h.data_ptr(), 0, 0,
+        assert L.kfr_amd_conv_backward(c, None, dh.data_ptr(), x.data_ptr(), x.data_ptr(), h.data_ptr(), 0, 0, 0,
                                        None) == 0
         torch.cuda.synchronize()
         assert torch.all(y == 3.0) and torch.all(dh == 0.0)
