@@ -273,6 +273,67 @@ KFR_AMD_API int kfr_amd_dft_permutation(int dtype, size_t size, int64_t* perm);
  * (include/kfr/dft/fft.hpp:389-391). */
 KFR_AMD_API size_t kfr_amd_dft_progressive_total_steps(int dtype, size_t size);
 
+/* Plan cache: the process-wide cache of dft_cache (include/kfr/dft/cache.hpp:44-126: get(ctype, size),
+ * getreal(ctype, size), clear()), keyed on element type, transform kind, size, pack format, batch
+ * layout and schedule variant rather than on type and size only. Each call returns a new reference to
+ * the cached plan, released with the matching kfr_dft_delete_plan_* / kfr_dft_real_delete_plan_*; the
+ * plan stays valid until then, also across kfr_amd_cache_clear (which drops the cache's own
+ * references). The convolutions (kfr_amd_convolve) take their plans from the same cache, as the
+ * reference's convolve does (src/dft/convolution-impl.cpp:106-128). */
+KFR_AMD_API KFR_DFT_PLAN_F32* kfr_amd_cache_dft_plan_f32(size_t size);
+KFR_AMD_API KFR_DFT_PLAN_F64* kfr_amd_cache_dft_plan_f64(size_t size);
+KFR_AMD_API KFR_DFT_REAL_PLAN_F32* kfr_amd_cache_dft_real_plan_f32(size_t size, KFR_DFT_PACK_FORMAT fmt);
+KFR_AMD_API KFR_DFT_REAL_PLAN_F64* kfr_amd_cache_dft_real_plan_f64(size_t size, KFR_DFT_PACK_FORMAT fmt);
+KFR_AMD_API void kfr_amd_cache_clear(void);
+KFR_AMD_API size_t kfr_amd_cache_size(void);
+
+/* convolve_filter<complex<T>> (include/kfr/dft/convolution.hpp:99-148 with a complex T; the reference's
+ * C API declares the KFR_FILTER_C32/C64 handles, capi.h:163-164, but no creator): streaming FFT
+ * convolution of complex signals with complex taps, the same contract as the real
+ * kfr_filter_*_convolution functions. set_data replaces the taps and resets the state, as
+ * convolve_filter::set_data does (src/dft/convolution-impl.cpp:58-84); the _f32/_f64 forms act on the
+ * real filters of kfr_filter_create_convolution_plan_*. */
+KFR_AMD_API KFR_FILTER_C32* kfr_amd_filter_create_convolution_plan_c32(const kfr_c32* taps, size_t size,
+                                                                      size_t block_size);
+KFR_AMD_API KFR_FILTER_C64* kfr_amd_filter_create_convolution_plan_c64(const kfr_c64* taps, size_t size,
+                                                                      size_t block_size);
+KFR_AMD_API void kfr_amd_filter_process_c32(KFR_FILTER_C32* plan, kfr_c32* output, const kfr_c32* input, size_t size);
+KFR_AMD_API void kfr_amd_filter_process_c64(KFR_FILTER_C64* plan, kfr_c64* output, const kfr_c64* input, size_t size);
+KFR_AMD_API void kfr_amd_filter_reset_c32(KFR_FILTER_C32* plan);
+KFR_AMD_API void kfr_amd_filter_reset_c64(KFR_FILTER_C64* plan);
+KFR_AMD_API void kfr_amd_filter_delete_plan_c32(KFR_FILTER_C32* plan);
+KFR_AMD_API void kfr_amd_filter_delete_plan_c64(KFR_FILTER_C64* plan);
+KFR_AMD_API void kfr_amd_filter_set_data_f32(KFR_FILTER_F32* plan, const kfr_f32* taps, size_t size);
+KFR_AMD_API void kfr_amd_filter_set_data_f64(KFR_FILTER_F64* plan, const kfr_f64* taps, size_t size);
+KFR_AMD_API void kfr_amd_filter_set_data_c32(KFR_FILTER_C32* plan, const kfr_c32* taps, size_t size);
+KFR_AMD_API void kfr_amd_filter_set_data_c64(KFR_FILTER_C64* plan, const kfr_c64* taps, size_t size);
+
+/* Progressive execution (dft_plan::progressive_start / progressive_step / progressive_total_steps,
+ * include/kfr/dft/fft.hpp:393-439, src/dft/fft-impl.hpp:2025-2064; dft_plan_real's progressive_start,
+ * fft.hpp:538-556). start records the buffers (host or device pointers) and returns a handle that keeps
+ * the plan alive; each kfr_amd_progressive_step runs the next step synchronously and returns 1 while
+ * steps remain, 0 after the last one (then `out` holds the result), -1 on error. The step count is the
+ * reference plan's stage count for that direction (progressive_optimized as in dft_plan's constructor);
+ * the transform's GPU launches (one Stockham pass each for lengths in one workgroup, the two kernels
+ * of a four-step, the whole transform for Bluestein, plus the real repack or unpack) are spread over
+ * the steps, the input is read at the first step and the output written at the last. */
+typedef struct KFR_AMD_PROGRESSIVE { int opaque; } KFR_AMD_PROGRESSIVE;
+KFR_AMD_API KFR_AMD_PROGRESSIVE* kfr_amd_dft_progressive_start_f32(KFR_DFT_PLAN_F32* plan, int progressive_optimized,
+                                                                   int inverse, kfr_c32* out, const kfr_c32* in);
+KFR_AMD_API KFR_AMD_PROGRESSIVE* kfr_amd_dft_progressive_start_f64(KFR_DFT_PLAN_F64* plan, int progressive_optimized,
+                                                                   int inverse, kfr_c64* out, const kfr_c64* in);
+KFR_AMD_API KFR_AMD_PROGRESSIVE* kfr_amd_dft_real_progressive_start_f32(KFR_DFT_REAL_PLAN_F32* plan, int inverse,
+                                                                        void* out, const void* in);
+KFR_AMD_API KFR_AMD_PROGRESSIVE* kfr_amd_dft_real_progressive_start_f64(KFR_DFT_REAL_PLAN_F64* plan, int inverse,
+                                                                        void* out, const void* in);
+KFR_AMD_API size_t kfr_amd_progressive_total_steps(KFR_AMD_PROGRESSIVE* prog);
+KFR_AMD_API int kfr_amd_progressive_step(KFR_AMD_PROGRESSIVE* prog);
+KFR_AMD_API void kfr_amd_progressive_delete(KFR_AMD_PROGRESSIVE* prog);
+/* dft_plan<T>(size, normal, progressive_optimized).progressive_total_steps() and the same for
+ * dft_plan_real<T>(size, fmt), computed on the host */
+KFR_AMD_API size_t kfr_amd_dft_progressive_steps(int dtype, size_t size, int progressive_optimized);
+KFR_AMD_API size_t kfr_amd_dft_real_progressive_steps(int dtype, size_t size, KFR_DFT_PACK_FORMAT fmt);
+
 /* The GPU schedule chosen for a complex DFT size (algorithm, radices, launch geometry). */
 KFR_AMD_API int kfr_amd_dft_schedule(int dtype, size_t size, char* buf, size_t buflen);
 

@@ -7,9 +7,10 @@ convolution with hand-written adjoint kernels (kfr_amd.autograd).
 """
 from ._lib import KfrError, last_error, lib
 from .dft import (CCs, Perm, autocorrelate, convolve, convolve_filter, correlate, dct_plan, describe, dft,
-                  dft_batch, dft_cache, dft_plan_half, dft_plan_md, dft_plan_md_real, dft_plan, dft_plan_real, idft, permutation, schedule)
+                  dft_batch, dft_cache, dft_plan_half, dft_plan_md, dft_plan_md_real, dft_plan, dft_plan_real, idft, irealdft,
+                  permutation, realdft, schedule)
 from .timing import kernel_timing
 
 __all__ = ["KfrError", "last_error", "lib", "CCs", "Perm", "autocorrelate", "convolve", "convolve_filter",
-           "correlate", "dct_plan", "describe", "dft", "dft_batch", "dft_cache", "dft_plan_half", "dft_plan_md", "dft_plan_md_real", "dft_plan", "dft_plan_real", "idft",
+           "correlate", "dct_plan", "describe", "dft", "dft_batch", "dft_cache", "dft_plan_half", "dft_plan_md", "dft_plan_md_real", "dft_plan", "dft_plan_real", "idft", "irealdft", "realdft",
            "kernel_timing", "permutation", "schedule"]

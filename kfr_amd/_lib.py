@@ -76,6 +76,25 @@ def _proto(lib):
         f(f"kfr_amd_dft_real_execute_batch_{t}", C.c_int, vp, vp, vp, c_size, C.c_int, vp)
         f(f"kfr_amd_dct_execute_batch_{t}", C.c_int, vp, vp, vp, c_size, C.c_int, vp)
         f(f"kfr_amd_dft_real_get_complex_size_{t}", c_size, vp)
+    for t in ("c32", "c64"):
+        f(f"kfr_amd_filter_create_convolution_plan_{t}", vp, vp, c_size, c_size)
+        f(f"kfr_amd_filter_process_{t}", None, vp, vp, vp, c_size)
+        f(f"kfr_amd_filter_reset_{t}", None, vp)
+        f(f"kfr_amd_filter_delete_plan_{t}", None, vp)
+    for t in ("f32", "f64", "c32", "c64"):
+        f(f"kfr_amd_filter_set_data_{t}", None, vp, vp, c_size)
+    for t in ("f32", "f64"):
+        f(f"kfr_amd_cache_dft_plan_{t}", vp, c_size)
+        f(f"kfr_amd_cache_dft_real_plan_{t}", vp, c_size, C.c_int)
+        f(f"kfr_amd_dft_progressive_start_{t}", vp, vp, C.c_int, C.c_int, vp, vp)
+        f(f"kfr_amd_dft_real_progressive_start_{t}", vp, vp, C.c_int, vp, vp)
+    f("kfr_amd_cache_clear", None)
+    f("kfr_amd_cache_size", c_size)
+    f("kfr_amd_progressive_total_steps", c_size, vp)
+    f("kfr_amd_progressive_step", C.c_int, vp)
+    f("kfr_amd_progressive_delete", None, vp)
+    f("kfr_amd_dft_progressive_steps", c_size, C.c_int, c_size, C.c_int)
+    f("kfr_amd_dft_real_progressive_steps", c_size, C.c_int, c_size, C.c_int)
     f("kfr_amd_convolve", C.c_long, C.c_int, C.c_int, vp, c_size, vp, c_size, vp)
     f("kfr_amd_conv_create", vp, C.c_int, c_size, c_size)
     f("kfr_amd_conv_fft_size", c_size, vp)

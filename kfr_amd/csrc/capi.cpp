@@ -12,6 +12,9 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
+#include <map>
+#include <tuple>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -169,10 +172,24 @@ struct stager
     }
 };
 
+// Plans are reference counted: the creator's handle holds one reference, the plan cache one per
+// cached plan, and a progressive execution one for its plan; kfr_*_delete_plan_* drops a reference.
+struct plan_ref
+{
+    std::atomic<int> refs{ 1 };
+    virtual ~plan_ref() = default;
+};
+void retain(plan_ref* p) { p->refs.fetch_add(1, std::memory_order_relaxed); }
+void release(plan_ref* p)
+{
+    if (p && p->refs.fetch_sub(1, std::memory_order_acq_rel) == 1)
+        delete p;
+}
+
 // a complex plan: one axis (dft_plan) or several (dft_plan_md), as the reference's C API keeps both
 // behind the same handle (src/capi/capi.cpp:96-173)
 template <typename T>
-struct dft_obj
+struct dft_obj : plan_ref
 {
     std::vector<plan_desc> desc;       // the reference plan of every axis
     std::unique_ptr<gpu_dft<T>> g;     // one axis
@@ -220,7 +237,7 @@ struct dft_obj
 
 // a real plan: one axis (dft_plan_real) or several (dft_plan_md_real, CCs along the last axis)
 template <typename T>
-struct real_obj
+struct real_obj : plan_ref
 {
     std::vector<plan_desc> desc;            // complex axes first, then the real last axis
     std::unique_ptr<gpu_dft_real<T>> g;     // one axis
@@ -296,7 +313,7 @@ struct real_obj
     }
 };
 template <typename T>
-struct dct_obj
+struct dct_obj : plan_ref
 {
     plan_desc desc;
     gpu_dct<T> g;
@@ -406,6 +423,80 @@ std::vector<size_t> shape_of(size_t dims, const unsigned* shape)
     throw std::logic_error(std::string(what) + " is not provided by this library (DFT path only)");
 }
 
+// ---- process-wide plan cache (the reference's dft_cache, include/kfr/dft/cache.hpp:44-126, keys only
+// on type and size); here the key also names the transform kind, the pack format, the batch layout and
+// the schedule variant, so that plans of one size but different kinds never alias
+struct cache_key
+{
+    int dtype   = 0; // 0 f32, 1 f64
+    int kind    = 0; // 0 complex, 1 real, 3 real convolution, 4 complex convolution
+    size_t n    = 0; // transform size (real convolution: signal length; complex convolution: FFT size)
+    size_t m    = 0; // real convolution: filter length
+    int fmt     = 0; // real pack format (0 Perm, 1 CCs)
+    int layout  = 0; // batch layout: 0 = contiguous items
+    int tuning  = 0; // schedule variant: 0 = the measured default
+    auto tie() const { return std::tie(dtype, kind, n, m, fmt, layout, tuning); }
+    bool operator<(const cache_key& o) const { return tie() < o.tie(); }
+};
+
+class plan_cache
+{
+public:
+    static plan_cache& get()
+    {
+        static plan_cache* c = new plan_cache; // never destroyed: plans outlive static destruction
+        return *c;
+    }
+    // a new reference to the cached object of key k, created by make() when absent
+    template <typename Obj, typename Make>
+    Obj* acquire(const cache_key& k, Make&& make)
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = map.find(k);
+        if (it == map.end())
+            it = map.emplace(k, static_cast<plan_ref*>(make())).first;
+        retain(it->second);
+        return static_cast<Obj*>(it->second);
+    }
+    void clear()
+    {
+        std::map<cache_key, plan_ref*> old;
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            old.swap(map);
+        }
+        for (auto& e : old)
+            release(e.second);
+    }
+    size_t size()
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        return map.size();
+    }
+
+private:
+    std::mutex mu;
+    std::map<cache_key, plan_ref*> map;
+};
+
+// one reference held for a scope
+struct ref_holder
+{
+    plan_ref* p;
+    ~ref_holder() { release(p); }
+};
+
+// convolution plans of the cache: the real convolution of two lengths, or the complex FFT of one size,
+// with their staging buffers (calls on one entry are serialised)
+template <typename T>
+struct conv_entry : plan_ref
+{
+    std::mutex mu;
+    std::unique_ptr<gpu_conv<T>> real;
+    std::unique_ptr<gpu_dft<T>> cplx;
+    dev_mem da, db, dy, A, B;
+};
+
 template <typename T>
 long convolve_impl(int mode, const void* a, size_t na, const void* b, size_t nb, void* out, bool complex_in)
 {
@@ -424,22 +515,191 @@ long convolve_impl(int mode, const void* a, size_t na, const void* b, size_t nb,
     const size_t nout = ny - off;
     if (!out)
         return (long)nout;
-    hipStream_t s = library_stream();
-    dev_mem da(na * es), db(nb * es), dy(ny * es);
-    hip_check(hipMemcpyAsync(da.ptr, a, na * es, hipMemcpyDefault, s), "convolve input copy");
-    hip_check(hipMemcpyAsync(db.ptr, b, nb * es, hipMemcpyDefault, s), "convolve input copy");
+    // the plans come from the cache (the reference's convolve takes its plan from dft_cache,
+    // src/dft/convolution-impl.cpp:106-128): repeated calls of the same sizes build no tables
+    hipStream_t s  = library_stream();
+    const size_t S = next_pow2(ny);
+    cache_key k;
+    k.dtype = sizeof(T) == 4 ? 0 : 1;
+    k.kind  = complex_in ? 4 : 3;
+    k.n     = complex_in ? S : na;
+    k.m     = complex_in ? 0 : nb;
+    auto* e = plan_cache::get().acquire<conv_entry<T>>(k,
+                                                       [&]
+                                                       {
+                                                           auto* c = new conv_entry<T>();
+                                                           try
+                                                           {
+                                                               if (complex_in)
+                                                                   c->cplx.reset(new gpu_dft<T>(S));
+                                                               else
+                                                                   c->real.reset(new gpu_conv<T>(na, nb));
+                                                           }
+                                                           catch (...)
+                                                           {
+                                                               delete c;
+                                                               throw;
+                                                           }
+                                                           return c;
+                                                       });
+    ref_holder hold{ e };
+    std::lock_guard<std::mutex> lock(e->mu);
+    e->da.ensure(na * es);
+    e->db.ensure(nb * es);
+    e->dy.ensure(ny * es);
+    hip_check(hipMemcpyAsync(e->da.ptr, a, na * es, hipMemcpyDefault, s), "convolve input copy");
+    hip_check(hipMemcpyAsync(e->db.ptr, b, nb * es, hipMemcpyDefault, s), "convolve input copy");
     if (complex_in)
-        gpu_convolve_complex<T>(dy.ptr, da.ptr, na, db.ptr, nb, correlate, false, s);
-    else
     {
-        gpu_conv<T> c(na, nb);
-        c.forward(dy.ptr, da.ptr, db.ptr, 0, 1, correlate, s);
+        e->A.ensure(S * es);
+        e->B.ensure(S * es);
+        gpu_convolve_complex<T>(e->dy.ptr, e->da.ptr, na, e->db.ptr, nb, correlate, false, *e->cplx, e->A.ptr, e->B.ptr,
+                                s);
     }
-    hip_check(hipMemcpyAsync(out, static_cast<char*>(dy.ptr) + off * es, nout * es, hipMemcpyDefault, s),
+    else
+        e->real->forward(e->dy.ptr, e->da.ptr, e->db.ptr, 0, 1, correlate, s);
+    hip_check(hipMemcpyAsync(out, static_cast<char*>(e->dy.ptr) + off * es, nout * es, hipMemcpyDefault, s),
               "convolve output copy");
     hip_check(hipStreamSynchronize(s), "convolve");
     return (long)nout;
 }
+
+// streaming filter of complex signals (convolve_filter<complex<T>>)
+template <typename T>
+struct cfilter_obj
+{
+    gpu_conv_filter_complex<T> g;
+    dev_mem in_buf, out_buf;
+    std::mutex mu;
+    cfilter_obj(const void* taps, size_t n, size_t block) : g(taps, n, block) {}
+};
+
+template <typename T>
+void cfilter_process(void* plan, void* out, const void* in, size_t n)
+{
+    check_plan(plan);
+    auto* f = static_cast<cfilter_obj<T>*>(plan);
+    std::lock_guard<std::mutex> lock(f->mu);
+    if (n == 0)
+        return;
+    hipStream_t s      = library_stream();
+    const size_t bytes = n * 2 * sizeof(T);
+    f->in_buf.ensure(bytes);
+    f->out_buf.ensure(bytes);
+    hip_check(hipMemcpyAsync(f->in_buf.ptr, in, bytes, hipMemcpyDefault, s), "filter input copy");
+    f->g.process(f->out_buf.ptr, f->in_buf.ptr, n, s);
+    hip_check(hipMemcpyAsync(out, f->out_buf.ptr, bytes, hipMemcpyDefault, s), "filter output copy");
+    hip_check(hipStreamSynchronize(s), "filter process");
+}
+
+// ---- progressive execution (include/kfr/dft/fft.hpp:393-439): progressive_total_steps() steps as the
+// reference counts them (its stage count for this plan and direction); the GPU units of the plan are
+// spread over the steps (unit u at step round(u (K-1) / (U-1)), a single unit at step 0), the input is
+// read at the first step and the output written at the last. Each step is synchronous.
+struct progressive_obj
+{
+    virtual ~progressive_obj() = default;
+    size_t total = 1, step = 0;
+    int units    = 1;
+    int unit_step(int u) const
+    {
+        if (units <= 1 || total <= 1)
+            return 0;
+        return (int)(((long long)u * (long long)(total - 1) * 2 + (units - 1)) / (2 * (long long)(units - 1)));
+    }
+    virtual void run_step(size_t i) = 0;
+};
+
+template <typename T>
+struct dft_progressive : progressive_obj
+{
+    dft_obj<T>* plan;
+    bool inverse;
+    void* out;
+    const void* in;
+    dev_mem W, mid;
+    dft_progressive(dft_obj<T>* p, bool prog_opt, bool inv, void* o, const void* i)
+        : plan(p), inverse(inv), out(o), in(i)
+    {
+        if (!plan->g)
+            throw std::logic_error("progressive execution is provided for one-dimensional plans");
+        const dtype_t dt = sizeof(T) == 4 ? dtype_t::f32 : dtype_t::f64;
+        total            = std::max<size_t>(1, describe_dft(dt, plan->size(), prog_opt).order[inverse ? 1 : 0].size());
+        units            = plan->g->progressive_prepare((int)total);
+        W.reset(plan->size() * 2 * sizeof(T));
+        if (size_t mb = plan->g->progressive_mid_bytes())
+            mid.reset(mb);
+        retain(plan);
+    }
+    ~dft_progressive() override { release(plan); }
+    void run_step(size_t i) override
+    {
+        hipStream_t s      = library_stream();
+        const size_t bytes = plan->size() * 2 * sizeof(T);
+        if (i == 0)
+            hip_check(hipDeviceSynchronize(), "progressive start"); // device inputs may still be in production
+        plan->ord.run(s,
+                      [&]
+                      {
+                          if (i == 0)
+                              hip_check(hipMemcpyAsync(W.ptr, in, bytes, hipMemcpyDefault, s), "progressive input copy");
+                          for (int u = 0; u < units; ++u)
+                              if ((size_t)unit_step(u) == i)
+                                  plan->g->progressive_unit(u, W.ptr, mid.ptr, inverse, s);
+                          if (i + 1 == total)
+                              hip_check(hipMemcpyAsync(out, W.ptr, bytes, hipMemcpyDefault, s), "progressive output copy");
+                      });
+        hip_check(hipStreamSynchronize(s), "progressive step");
+    }
+};
+
+template <typename T>
+struct real_progressive : progressive_obj
+{
+    real_obj<T>* plan;
+    bool inverse;
+    void* out;
+    const void* in;
+    dev_mem Z, Cb, mid;
+    real_progressive(real_obj<T>* p, bool inv, void* o, const void* i) : plan(p), inverse(inv), out(o), in(i)
+    {
+        if (!plan->g)
+            throw std::logic_error("progressive execution is provided for one-dimensional plans");
+        const dtype_t dt = sizeof(T) == 4 ? dtype_t::f32 : dtype_t::f64;
+        total = std::max<size_t>(1, describe_dft_real(dt, plan->size(), plan->g->format()).order[inverse ? 1 : 0].size());
+        units = plan->g->progressive_prepare((int)total);
+        Z.reset(plan->size() * sizeof(T));
+        Cb.reset(plan->complex_size() * 2 * sizeof(T));
+        if (size_t mb = plan->g->progressive_mid_bytes())
+            mid.reset(mb);
+        retain(plan);
+    }
+    ~real_progressive() override { release(plan); }
+    void run_step(size_t i) override
+    {
+        hipStream_t s     = library_stream();
+        const size_t rb   = plan->size() * sizeof(T);
+        const size_t cb   = plan->complex_size() * 2 * sizeof(T);
+        if (i == 0)
+            hip_check(hipDeviceSynchronize(), "progressive start");
+        plan->ord.run(s,
+                      [&]
+                      {
+                          if (i == 0)
+                              hip_check(hipMemcpyAsync(inverse ? Cb.ptr : Z.ptr, in, inverse ? cb : rb, hipMemcpyDefault,
+                                                       s),
+                                        "progressive input copy");
+                          for (int u = 0; u < units; ++u)
+                              if ((size_t)unit_step(u) == i)
+                                  plan->g->progressive_unit(u, Z.ptr, Cb.ptr, mid.ptr, inverse, s);
+                          if (i + 1 == total)
+                              hip_check(hipMemcpyAsync(out, inverse ? Z.ptr : Cb.ptr, inverse ? rb : cb, hipMemcpyDefault,
+                                                       s),
+                                        "progressive output copy");
+                      });
+        hip_check(hipStreamSynchronize(s), "progressive step");
+    }
+};
 
 struct conv_obj
 {
@@ -617,11 +877,11 @@ extern "C"
     }
     void kfr_dft_delete_plan_f32(KFR_DFT_PLAN_F32* plan)
     {
-        try_void([&] { delete reinterpret_cast<dft_obj<float>*>(plan); });
+        try_void([&] { release(reinterpret_cast<dft_obj<float>*>(plan)); });
     }
     void kfr_dft_delete_plan_f64(KFR_DFT_PLAN_F64* plan)
     {
-        try_void([&] { delete reinterpret_cast<dft_obj<double>*>(plan); });
+        try_void([&] { release(reinterpret_cast<dft_obj<double>*>(plan)); });
     }
 
     // ---- real DFT
@@ -725,11 +985,11 @@ extern "C"
     }
     void kfr_dft_real_delete_plan_f32(KFR_DFT_REAL_PLAN_F32* plan)
     {
-        try_void([&] { delete reinterpret_cast<real_obj<float>*>(plan); });
+        try_void([&] { release(reinterpret_cast<real_obj<float>*>(plan)); });
     }
     void kfr_dft_real_delete_plan_f64(KFR_DFT_REAL_PLAN_F64* plan)
     {
-        try_void([&] { delete reinterpret_cast<real_obj<double>*>(plan); });
+        try_void([&] { release(reinterpret_cast<real_obj<double>*>(plan)); });
     }
 
     // ---- DCT
@@ -785,11 +1045,11 @@ extern "C"
     }
     void kfr_dct_delete_plan_f32(KFR_DCT_PLAN_F32* plan)
     {
-        try_void([&] { delete reinterpret_cast<dct_obj<float>*>(plan); });
+        try_void([&] { release(reinterpret_cast<dct_obj<float>*>(plan)); });
     }
     void kfr_dct_delete_plan_f64(KFR_DCT_PLAN_F64* plan)
     {
-        try_void([&] { delete reinterpret_cast<dct_obj<double>*>(plan); });
+        try_void([&] { release(reinterpret_cast<dct_obj<double>*>(plan)); });
     }
 
     // ---- filters
@@ -1342,6 +1602,248 @@ extern "C"
                 return 0;
             },
             -1);
+    }
+
+    // ---- plan cache
+    KFR_DFT_PLAN_F32* kfr_amd_cache_dft_plan_f32(size_t size)
+    {
+        return try_fn(
+            [&]
+            {
+                cache_key k;
+                k.dtype = 0;
+                k.n     = size;
+                return reinterpret_cast<KFR_DFT_PLAN_F32*>(plan_cache::get().acquire<dft_obj<float>>(
+                    k, [&] { return new dft_obj<float>(std::vector<size_t>{ size }); }));
+            },
+            (KFR_DFT_PLAN_F32*)nullptr);
+    }
+    KFR_DFT_PLAN_F64* kfr_amd_cache_dft_plan_f64(size_t size)
+    {
+        return try_fn(
+            [&]
+            {
+                cache_key k;
+                k.dtype = 1;
+                k.n     = size;
+                return reinterpret_cast<KFR_DFT_PLAN_F64*>(plan_cache::get().acquire<dft_obj<double>>(
+                    k, [&] { return new dft_obj<double>(std::vector<size_t>{ size }); }));
+            },
+            (KFR_DFT_PLAN_F64*)nullptr);
+    }
+    KFR_DFT_REAL_PLAN_F32* kfr_amd_cache_dft_real_plan_f32(size_t size, KFR_DFT_PACK_FORMAT fmt)
+    {
+        return try_fn(
+            [&]
+            {
+                cache_key k;
+                k.dtype = 0;
+                k.kind  = 1;
+                k.n     = size;
+                k.fmt   = (int)fmt;
+                return reinterpret_cast<KFR_DFT_REAL_PLAN_F32*>(plan_cache::get().acquire<real_obj<float>>(
+                    k, [&] { return new real_obj<float>(size, (int)fmt); }));
+            },
+            (KFR_DFT_REAL_PLAN_F32*)nullptr);
+    }
+    KFR_DFT_REAL_PLAN_F64* kfr_amd_cache_dft_real_plan_f64(size_t size, KFR_DFT_PACK_FORMAT fmt)
+    {
+        return try_fn(
+            [&]
+            {
+                cache_key k;
+                k.dtype = 1;
+                k.kind  = 1;
+                k.n     = size;
+                k.fmt   = (int)fmt;
+                return reinterpret_cast<KFR_DFT_REAL_PLAN_F64*>(plan_cache::get().acquire<real_obj<double>>(
+                    k, [&] { return new real_obj<double>(size, (int)fmt); }));
+            },
+            (KFR_DFT_REAL_PLAN_F64*)nullptr);
+    }
+    void kfr_amd_cache_clear(void)
+    {
+        try_void([] { plan_cache::get().clear(); });
+    }
+    size_t kfr_amd_cache_size(void)
+    {
+        return try_fn([] { return plan_cache::get().size(); }, (size_t)0);
+    }
+
+    // ---- complex streaming filters
+    KFR_FILTER_C32* kfr_amd_filter_create_convolution_plan_c32(const kfr_c32* taps, size_t size, size_t block_size)
+    {
+        return try_fn(
+            [&] { return reinterpret_cast<KFR_FILTER_C32*>(new cfilter_obj<float>(taps, size, block_size ? block_size : 1024)); },
+            (KFR_FILTER_C32*)nullptr);
+    }
+    KFR_FILTER_C64* kfr_amd_filter_create_convolution_plan_c64(const kfr_c64* taps, size_t size, size_t block_size)
+    {
+        return try_fn(
+            [&]
+            { return reinterpret_cast<KFR_FILTER_C64*>(new cfilter_obj<double>(taps, size, block_size ? block_size : 1024)); },
+            (KFR_FILTER_C64*)nullptr);
+    }
+    void kfr_amd_filter_process_c32(KFR_FILTER_C32* plan, kfr_c32* output, const kfr_c32* input, size_t size)
+    {
+        try_void([&] { cfilter_process<float>(plan, output, input, size); });
+    }
+    void kfr_amd_filter_process_c64(KFR_FILTER_C64* plan, kfr_c64* output, const kfr_c64* input, size_t size)
+    {
+        try_void([&] { cfilter_process<double>(plan, output, input, size); });
+    }
+    void kfr_amd_filter_reset_c32(KFR_FILTER_C32* plan)
+    {
+        try_void(
+            [&]
+            {
+                check_plan(plan);
+                reinterpret_cast<cfilter_obj<float>*>(plan)->g.reset();
+            });
+    }
+    void kfr_amd_filter_reset_c64(KFR_FILTER_C64* plan)
+    {
+        try_void(
+            [&]
+            {
+                check_plan(plan);
+                reinterpret_cast<cfilter_obj<double>*>(plan)->g.reset();
+            });
+    }
+    void kfr_amd_filter_delete_plan_c32(KFR_FILTER_C32* plan)
+    {
+        try_void([&] { delete reinterpret_cast<cfilter_obj<float>*>(plan); });
+    }
+    void kfr_amd_filter_delete_plan_c64(KFR_FILTER_C64* plan)
+    {
+        try_void([&] { delete reinterpret_cast<cfilter_obj<double>*>(plan); });
+    }
+    void kfr_amd_filter_set_data_f32(KFR_FILTER_F32* plan, const kfr_f32* taps, size_t size)
+    {
+        try_void(
+            [&]
+            {
+                check_plan(plan);
+                auto* f = reinterpret_cast<filter_obj<float>*>(plan);
+                std::lock_guard<std::mutex> lock(f->mu);
+                f->g.set_data(taps, size);
+            });
+    }
+    void kfr_amd_filter_set_data_f64(KFR_FILTER_F64* plan, const kfr_f64* taps, size_t size)
+    {
+        try_void(
+            [&]
+            {
+                check_plan(plan);
+                auto* f = reinterpret_cast<filter_obj<double>*>(plan);
+                std::lock_guard<std::mutex> lock(f->mu);
+                f->g.set_data(taps, size);
+            });
+    }
+    void kfr_amd_filter_set_data_c32(KFR_FILTER_C32* plan, const kfr_c32* taps, size_t size)
+    {
+        try_void(
+            [&]
+            {
+                check_plan(plan);
+                auto* f = reinterpret_cast<cfilter_obj<float>*>(plan);
+                std::lock_guard<std::mutex> lock(f->mu);
+                f->g.set_data(taps, size);
+            });
+    }
+    void kfr_amd_filter_set_data_c64(KFR_FILTER_C64* plan, const kfr_c64* taps, size_t size)
+    {
+        try_void(
+            [&]
+            {
+                check_plan(plan);
+                auto* f = reinterpret_cast<cfilter_obj<double>*>(plan);
+                std::lock_guard<std::mutex> lock(f->mu);
+                f->g.set_data(taps, size);
+            });
+    }
+
+    // ---- progressive execution
+    KFR_AMD_PROGRESSIVE* kfr_amd_dft_progressive_start_f32(KFR_DFT_PLAN_F32* plan, int progressive_optimized,
+                                                           int inverse, kfr_c32* out, const kfr_c32* in)
+    {
+        return try_fn(
+            [&]
+            {
+                return reinterpret_cast<KFR_AMD_PROGRESSIVE*>(static_cast<progressive_obj*>(
+                    new dft_progressive<float>(dft_cast<float>(plan), progressive_optimized != 0, inverse != 0, out, in)));
+            },
+            (KFR_AMD_PROGRESSIVE*)nullptr);
+    }
+    KFR_AMD_PROGRESSIVE* kfr_amd_dft_progressive_start_f64(KFR_DFT_PLAN_F64* plan, int progressive_optimized,
+                                                           int inverse, kfr_c64* out, const kfr_c64* in)
+    {
+        return try_fn(
+            [&]
+            {
+                return reinterpret_cast<KFR_AMD_PROGRESSIVE*>(static_cast<progressive_obj*>(new dft_progressive<double>(
+                    dft_cast<double>(plan), progressive_optimized != 0, inverse != 0, out, in)));
+            },
+            (KFR_AMD_PROGRESSIVE*)nullptr);
+    }
+    KFR_AMD_PROGRESSIVE* kfr_amd_dft_real_progressive_start_f32(KFR_DFT_REAL_PLAN_F32* plan, int inverse, void* out,
+                                                                const void* in)
+    {
+        return try_fn(
+            [&]
+            {
+                return reinterpret_cast<KFR_AMD_PROGRESSIVE*>(static_cast<progressive_obj*>(
+                    new real_progressive<float>(real_cast<float>(plan), inverse != 0, out, in)));
+            },
+            (KFR_AMD_PROGRESSIVE*)nullptr);
+    }
+    KFR_AMD_PROGRESSIVE* kfr_amd_dft_real_progressive_start_f64(KFR_DFT_REAL_PLAN_F64* plan, int inverse, void* out,
+                                                                const void* in)
+    {
+        return try_fn(
+            [&]
+            {
+                return reinterpret_cast<KFR_AMD_PROGRESSIVE*>(static_cast<progressive_obj*>(
+                    new real_progressive<double>(real_cast<double>(plan), inverse != 0, out, in)));
+            },
+            (KFR_AMD_PROGRESSIVE*)nullptr);
+    }
+    size_t kfr_amd_progressive_total_steps(KFR_AMD_PROGRESSIVE* prog)
+    {
+        return try_fn(
+            [&]
+            {
+                check_plan(prog);
+                return reinterpret_cast<progressive_obj*>(prog)->total;
+            },
+            (size_t)0);
+    }
+    int kfr_amd_progressive_step(KFR_AMD_PROGRESSIVE* prog)
+    {
+        return try_fn(
+            [&]
+            {
+                check_plan(prog);
+                auto* p = reinterpret_cast<progressive_obj*>(prog);
+                if (p->step >= p->total)
+                    throw std::logic_error("progressive execution is already complete");
+                p->run_step(p->step);
+                return ++p->step < p->total ? 1 : 0;
+            },
+            -1);
+    }
+    void kfr_amd_progressive_delete(KFR_AMD_PROGRESSIVE* prog)
+    {
+        try_void([&] { delete reinterpret_cast<progressive_obj*>(prog); });
+    }
+    size_t kfr_amd_dft_progressive_steps(int dtype, size_t size, int progressive_optimized)
+    {
+        return try_fn([&] { return describe_dft(to_dtype(dtype), size, progressive_optimized != 0).order[0].size(); },
+                      (size_t)0);
+    }
+    size_t kfr_amd_dft_real_progressive_steps(int dtype, size_t size, KFR_DFT_PACK_FORMAT fmt)
+    {
+        return try_fn([&] { return describe_dft_real(to_dtype(dtype), size, (int)fmt).order[0].size(); }, (size_t)0);
     }
 
     int kfr_amd_synchronize(void)

@@ -303,6 +303,8 @@ void gen_twiddles(cx<T>* out, long long count, long long M, long long mult, long
 {
     if (count <= 0)
         return;
+    timing::span ts([] { return timing::kernel_name("k_twiddles", timing::type_name<T>()); },
+                    (long long)sizeof(cx<T>) * count, s);
     hipLaunchKernelGGL(k_twiddles<T>, dim3(nblocks(count, 256)), dim3(256), 0, s, out, count, M, mult, add, mode,
                        R);
     hip_check(hipGetLastError(), "twiddle generation launch");
@@ -322,58 +324,16 @@ struct stockham
 
 inline bool lds_radix(int r) { return r == 2 || r == 3 || r == 4 || r == 5 || r == 7 || r == 8 || r == 16; }
 
-// Factor n into GPU passes (wide: the ceil(pm/R) butterflies per thread of radix_pass<..., WIDE>). Power-of-2 part: the largest butterfly that fits a thread's pm values
-// (16 for f32, 8 for f64) first, remainder 8/4/2; then 7, 5, 3; any other prime factor up to 127
-// becomes a direct DFT pass, placed last.
-inline bool make_stockham(long long nn, stockham& S, int pm, bool wide = false)
+// Pass geometry of an explicit radix list (product n): strides, twiddle offsets, threads per transform
+// (wide: the ceil(pm/R) butterflies per thread of radix_pass<..., WIDE>)
+inline bool stockham_from_radices(long long nn, const std::vector<int>& radices, stockham& S, int pm, bool wide = false)
 {
     S = stockham{};
-    if (nn < 1 || nn > (1 << 28))
+    if (nn < 1 || nn > (1 << 28) || (int)radices.size() > max_passes)
         return false;
     const int n = (int)nn;
     S.n         = n;
-    int m       = n;
-    std::vector<int> pw2, odd, gen;
-    int a = 0;
-    while (m % 2 == 0)
-    {
-        m /= 2;
-        ++a;
-    }
-    const int rbits = pm >= 16 ? 4 : 3;
-    while (a >= rbits)
-    {
-        pw2.push_back(1 << rbits);
-        a -= rbits;
-    }
-    if (a == 3)
-        pw2.push_back(8);
-    else if (a == 2)
-        pw2.push_back(4);
-    else if (a == 1)
-        pw2.push_back(2);
-    for (int p : { 7, 5, 3 })
-        while (m % p == 0)
-        {
-            odd.push_back(p);
-            m /= p;
-        }
-    for (int p = 11; m > 1 && p <= 127; p += 2)
-        while (m % p == 0)
-        {
-            gen.push_back(p);
-            m /= p;
-        }
-    if (m != 1)
-        return false;
-    for (int r : pw2)
-        S.radix.push_back(r);
-    for (int r : odd)
-        S.radix.push_back(r);
-    for (int r : gen)
-        S.radix.push_back(r);
-    if ((int)S.radix.size() > max_passes)
-        return false;
+    S.radix     = radices;
     long long ns = 1, off = 0;
     int tpt      = 1;
     for (int r : S.radix)
@@ -397,9 +357,87 @@ inline bool make_stockham(long long nn, stockham& S, int pm, bool wide = false)
         }
         ns *= r;
     }
+    if (ns != nn)
+        return false;
     S.tw_count = off;
     S.tpt      = tpt;
     return true;
+}
+
+// prime factors of m other than 2 (7, 5, 3 first, then 11..127 ascending); false if another remains
+inline bool odd_factors(long long& m, std::vector<int>& odd)
+{
+    for (int p : { 7, 5, 3 })
+        while (m % p == 0)
+        {
+            odd.push_back(p);
+            m /= p;
+        }
+    for (int p = 11; m > 1 && p <= 127; p += 2)
+        while (m % p == 0)
+        {
+            odd.push_back(p);
+            m /= p;
+        }
+    return m == 1;
+}
+
+// Factor n into GPU passes (wide: the ceil(pm/R) butterflies per thread of radix_pass<..., WIDE>).
+// Power-of-2 part: the largest butterfly that fits a thread's pm values (16 for f32, 8 for f64) first,
+// remainder 8/4/2; then 7, 5, 3; any other prime factor up to 127 becomes a direct DFT pass, placed last.
+inline bool make_stockham(long long nn, stockham& S, int pm, bool wide = false)
+{
+    S = stockham{};
+    if (nn < 1 || nn > (1 << 28))
+        return false;
+    long long m = nn;
+    int a       = 0;
+    while (m % 2 == 0)
+    {
+        m /= 2;
+        ++a;
+    }
+    std::vector<int> radix;
+    const int rbits = pm >= 16 ? 4 : 3;
+    while (a >= rbits)
+    {
+        radix.push_back(1 << rbits);
+        a -= rbits;
+    }
+    if (a > 0)
+        radix.push_back(1 << a);
+    if (!odd_factors(m, radix))
+        return false;
+    return stockham_from_radices(nn, radix, S, pm, wide);
+}
+
+// A finer schedule of the same transform for progressive execution, about `want` passes: every odd
+// prime factor its own pass, the power-of-two part split into near-equal passes (at most 2^4 points
+// for f32, 2^3 for f64, at least 2) so that the pass count approaches `want`.
+inline bool make_stockham_progressive(long long nn, int want, stockham& S, int pm)
+{
+    long long m = nn;
+    int a       = 0;
+    while (m % 2 == 0)
+    {
+        m /= 2;
+        ++a;
+    }
+    std::vector<int> odd;
+    if (!odd_factors(m, odd))
+        return false;
+    const int rbits = pm >= 16 ? 4 : 3;
+    int np2         = 0;
+    if (a > 0)
+        np2 = std::min(a, std::max((a + rbits - 1) / rbits, want - (int)odd.size()));
+    std::vector<int> radix;
+    for (int q = 0; q < np2; ++q)
+    {
+        const int bits = a / np2 + (q < a % np2 ? 1 : 0);
+        radix.push_back(1 << bits);
+    }
+    radix.insert(radix.end(), odd.begin(), odd.end());
+    return stockham_from_radices(nn, radix, S, pm);
 }
 
 template <typename T>
@@ -936,12 +974,22 @@ struct core_fft
             return;
         }
         scratch.ensure((size_t)std::min<long long>(chunk, batch) * n * sizeof(cx<T>));
+        run_fourstep<INV>(ld, st, batch, scratch.as<cx<T>>(), 3, s);
+    }
+
+    // four-step in rounds of at most `chunk` items through `mid`: phases bit 0 runs the column kernels,
+    // bit 1 the row kernels (one of them alone only within one round)
+    template <bool INV>
+    void run_fourstep(const xload<T>& ld, const xstore<T>& st, long long batch, cx<T>* mid, int phases,
+                      hipStream_t s)
+    {
+        const cx<T>* t   = tw.as<cx<T>>();
         lds_geom GA      = make_geom(SA, C, 1, C, 1);
         lds_geom GB      = make_geom(SB, R, 0, 1, (int)n1 + 1);
         const size_t shA = lds_padded_elems((size_t)C * n2) * sizeof(cx<T>);
         const size_t shB = lds_padded_elems((size_t)R * (n1 + 1)) * sizeof(cx<T>);
         fourstep_args<T> fs;
-        fs.scratch = scratch.as<cx<T>>();
+        fs.scratch = mid;
         fs.ta      = t + twTA;
         fs.tb      = t + twTB;
         fs.n1div   = make_fastdiv((uint32_t)n1);
@@ -960,13 +1008,56 @@ struct core_fft
             sb.b0 += b0;
             const long long ga = nb * (long long)n1;
             fs.ndiv            = make_fastdiv((uint32_t)n2);
-            launch_fft<T, INV, role_col>(mA, GA, t + twA, ga, la, st, fs, nblocks(ga, C), (unsigned)(SA.tpt * C), shA,
-                                         s);
+            if (phases & 1)
+                launch_fft<T, INV, role_col>(mA, GA, t + twA, ga, la, st, fs, nblocks(ga, C), (unsigned)(SA.tpt * C),
+                                             shA, s);
             const long long gb = nb * (long long)n2;
             fs.ndiv            = make_fastdiv((uint32_t)n1);
-            launch_fft<T, INV, role_row>(mB, GB, t + twB, gb, ld, sb, fs, nblocks(gb, R), (unsigned)(SB.tpt * R), shB,
-                                         s);
+            if (phases & 2)
+                launch_fft<T, INV, role_row>(mB, GB, t + twB, gb, ld, sb, fs, nblocks(gb, R), (unsigned)(SB.tpt * R),
+                                             shB, s);
         }
+    }
+
+    // ---- progressive execution (one reference stage per call, include/kfr/dft/fft.hpp:393-439): one
+    // item in place on `data`, as units of one launch each. Lengths in one workgroup run one pass of a
+    // finer Stockham schedule per unit (about `want` passes); four-step lengths run their column and
+    // row kernels as two units through `mid` (n values, kept apart from the plan's scratch).
+    stockham Sp;
+    dev_mem twp;
+    int tpbp = 1;
+    void progressive_prepare(int want, hipStream_t s)
+    {
+        if (fourstep || !Sp.radix.empty())
+            return;
+        if (!make_stockham_progressive((long long)n, std::max(1, want), Sp, pmax_of<T>))
+            throw std::logic_error("no progressive schedule for this length");
+        check_pass_radices<T>(Sp, is_mixed(Sp));
+        if (Sp.tpt > max_block_of(is_mixed(Sp)))
+            throw std::logic_error("progressive schedule needs too many threads per transform");
+        tpbp = 1; // one item per progressive execution
+        twp.reset((size_t)std::max<long long>(1, Sp.tw_count) * sizeof(cx<T>));
+        gen_stockham_twiddles(Sp, twp.as<cx<T>>(), s);
+    }
+    int progressive_units() const { return fourstep ? 2 : (int)Sp.radix.size(); }
+    template <bool INV>
+    void progressive_unit(int u, cx<T>* data, cx<T>* mid, hipStream_t s)
+    {
+        if (fourstep)
+        {
+            run_fourstep<INV>(load_plain<T>(data, (long long)n), store_plain<T>(data, (long long)n), 1, mid,
+                              u == 0 ? 1 : 2, s);
+            return;
+        }
+        lds_geom G = make_geom(Sp, tpbp, 0, 1, (int)n);
+        G.pass[0]  = G.pass[u];
+        G.np       = 1;
+        const size_t sh = lds_padded_elems((size_t)tpbp * n) * sizeof(cx<T>);
+        fourstep_args<T> fs;
+        fs.ndiv = make_fastdiv((uint32_t)n);
+        launch_fft<T, INV, role_direct>(is_mixed(Sp), G, twp.as<cx<T>>(), 1, load_plain<T>(data, (long long)n),
+                                        store_plain<T>(data, (long long)n), fs, 1, (unsigned)(Sp.tpt * tpbp), sh,
+                                        s);
     }
 
     // contiguous batch: item b at b*n, no element operation

@@ -349,6 +349,31 @@ struct gpu_dft<T>::impl
         else
             run_t<false>(ld, st, batch, s);
     }
+
+    // progressive execution (see gpu_dft::progressive_prepare)
+    int prog_prepare(int want)
+    {
+        if (algo != CORE)
+            return 1; // Bluestein: the whole transform in one unit
+        hipStream_t s = library_stream();
+        core->progressive_prepare(want, s);
+        hip_check(hipStreamSynchronize(s), "progressive plan");
+        return core->progressive_units();
+    }
+    int prog_units() const { return algo == CORE ? core->progressive_units() : 1; }
+    size_t prog_mid_bytes() const { return algo == CORE && core->fourstep ? n * sizeof(cx<T>) : 0; }
+    void prog_unit(int u, void* data, void* mid, bool inverse, hipStream_t s)
+    {
+        if (algo != CORE)
+        {
+            run(load_plain<T>(data, (long long)n), store_plain<T>(data, (long long)n), 1, inverse, s);
+            return;
+        }
+        if (inverse)
+            core->template progressive_unit<true>(u, static_cast<cx<T>*>(data), static_cast<cx<T>*>(mid), s);
+        else
+            core->template progressive_unit<false>(u, static_cast<cx<T>*>(data), static_cast<cx<T>*>(mid), s);
+    }
 };
 
 template <typename T>
@@ -381,6 +406,22 @@ template <typename T>
 void gpu_dft<T>::execute(void* out, const void* in, long long batch, bool inverse, hipStream_t s)
 {
     execute_dist(out, (long long)p->n, in, (long long)p->n, batch, inverse, s);
+}
+
+template <typename T>
+int gpu_dft<T>::progressive_prepare(int want)
+{
+    return p->prog_prepare(want);
+}
+template <typename T>
+size_t gpu_dft<T>::progressive_mid_bytes() const
+{
+    return p->prog_mid_bytes();
+}
+template <typename T>
+void gpu_dft<T>::progressive_unit(int u, void* data, void* mid, bool inverse, hipStream_t s)
+{
+    p->prog_unit(u, data, mid, inverse, s);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -598,6 +639,52 @@ void gpu_dft_real<T>::inverse(void* rout, const void* cin, long long batch, hipS
 {
     xload<T> src = load_plain<T>(cin, (long long)p->complex_size());
     p->inverse_op(src, store_plain<T>(rout, (long long)p->csize), batch, s);
+}
+
+template <typename T>
+int gpu_dft_real<T>::progressive_prepare(int want)
+{
+    return p->c->prog_prepare(std::max(1, want - 1)) + 1;
+}
+template <typename T>
+size_t gpu_dft_real<T>::progressive_mid_bytes() const
+{
+    return p->c->prog_mid_bytes();
+}
+template <typename T>
+void gpu_dft_real<T>::progressive_unit(int u, void* z, void* c, void* mid, bool inverse, hipStream_t s)
+{
+    const int half      = p->c->prog_units();
+    const long long cs  = (long long)p->csize;
+    if (!inverse)
+    {
+        if (u < half)
+        {
+            p->c->prog_unit(u, z, mid, false, s);
+            return;
+        }
+        // to_fmt repack of the half spectrum (src/dft/fft-impl.hpp:1799-1841)
+        const long long count = (cs + 1) / 2;
+        hipLaunchKernelGGL(k_real_repack<T>, dim3(nblocks(count + 1, 256)), dim3(256), 0, s, static_cast<cx<T>*>(c),
+                           (long long)p->complex_size(), static_cast<const cx<T>*>(z), cs, p->rtw.template as<cx<T>>(),
+                           cs, p->fmt, 1ll);
+        hip_check(hipGetLastError(), "real repack launch");
+        return;
+    }
+    if (u == 0)
+    {
+        // from_fmt unpacking (src/dft/fft-impl.hpp:1853-1896) into the half-length sequence
+        xload<T> src = load_plain<T>(c, (long long)p->complex_size());
+        src.mode     = ld_c2r;
+        src.r        = p->rtw.template as<cx<T>>();
+        src.n        = cs;
+        src.flag     = p->fmt;
+        hipLaunchKernelGGL(k_bs_gather<T>, dim3(nblocks(cs, 256)), dim3(256), 0, s, static_cast<cx<T>*>(z), cs, src,
+                           cs, 1ll);
+        hip_check(hipGetLastError(), "real unpack launch");
+        return;
+    }
+    p->c->prog_unit(u - 1, z, mid, true, s);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1077,25 +1164,26 @@ void gpu_conv<T>::backward(void* dx, void* dh, const void* g, const void* x, con
 // reference does (src/dft/convolution-impl.cpp:124-127)
 template <typename T>
 void gpu_convolve_complex(void* y, const void* a, size_t na, const void* b, size_t nb, bool correlate,
-                          bool keep_imag, hipStream_t s)
+                          bool keep_imag, gpu_dft<T>& plan, void* A, void* B, hipStream_t s)
 {
     const size_t ny = na + nb - 1;
     const size_t S  = next_pow2(ny);
-    typename gpu_dft<T>::impl f(S);
-    dev_mem A(S * sizeof(cx<T>)), B(S * sizeof(cx<T>));
+    if (plan.size() != S)
+        throw std::logic_error("complex convolution: plan size does not match");
+    typename gpu_dft<T>::impl& f = *plan.p;
     xload<T> la;
     la.mode = ld_cpad;
     la.p    = a;
     la.n    = (long long)na;
-    f.run(la, store_plain<T>(A.ptr, 0), 1, false, s);
+    f.run(la, store_plain<T>(A, 0), 1, false, s);
     xload<T> lb = la;
     lb.p        = b;
     lb.n        = (long long)nb;
     lb.flag     = correlate ? 1 : 0;
-    f.run(lb, store_plain<T>(B.ptr, 0), 1, false, s);
-    xload<T> lm = load_plain<T>(A.ptr, 0);
+    f.run(lb, store_plain<T>(B, 0), 1, false, s);
+    xload<T> lm = load_plain<T>(A, 0);
     lm.mode     = ld_mul;
-    lm.q        = B.as<cx<T>>();
+    lm.q        = static_cast<const cx<T>*>(B);
     xstore<T> sy;
     sy.mode  = st_ctrunc;
     sy.p     = y;
@@ -1123,7 +1211,7 @@ __global__ void k_gather_segments(T* seg, const T* hist, const T* in, long long 
         return;
     const long long c = idx / S, j = idx - c * S;
     const long long pos = c * L + j; // position in [history | input]
-    T v                 = T(0);
+    T v{};
     if (pos < nh1)
         v = hist[pos];
     else if (pos - nh1 < n)
@@ -1159,7 +1247,7 @@ __global__ void k_zero(T* p, long long count)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count)
-        p[i] = T(0);
+        p[i] = T{};
 }
 
 template <typename T>
@@ -1236,6 +1324,116 @@ struct gpu_conv_filter<T>::impl
     }
 };
 
+// complex signals and taps: the same overlap-save segments through complex S-point transforms, the
+// filter spectrum FFT(h)/S premultiplied into the load of the inverse transform
+template <typename T>
+struct gpu_conv_filter_complex<T>::impl
+{
+    size_t block = 1024, ntaps = 0, nh1 = 0, S = 0, L = 0;
+    std::unique_ptr<typename gpu_dft<T>::impl> f;
+    dev_mem H, hist[2], seg, yseg, taps_dev;
+    int cur = 0;
+
+    impl(const void* taps, size_t nt, size_t bs)
+    {
+        block = next_pow2(bs == 0 ? 1024 : bs);
+        set(taps, nt);
+    }
+    void set(const void* taps, size_t nt)
+    {
+        if (nt == 0)
+            throw std::logic_error("convolution filter needs at least one tap");
+        ntaps = nt;
+        nh1   = nt - 1;
+        S     = std::max<size_t>(next_pow2(std::max<size_t>(2 * block, 2 * nh1 + 2)), 2);
+        L     = S - nh1;
+        f.reset(new typename gpu_dft<T>::impl(S));
+        hipStream_t s = library_stream();
+        taps_dev.reset(nt * sizeof(cx<T>));
+        hip_check(hipMemcpy(taps_dev.ptr, taps, nt * sizeof(cx<T>), hipMemcpyHostToDevice), "filter taps upload");
+        H.reset(S * sizeof(cx<T>));
+        xload<T> lh;
+        lh.mode = ld_cpad;
+        lh.p    = taps_dev.ptr;
+        lh.n    = (long long)nt;
+        xstore<T> sh = store_plain<T>(H.ptr, 0);
+        f->run(lh, sh, 1, false, s);
+        gpu_scale<T>(H.ptr, 2 * S, T(1) / T(S), s);
+        hist[0].reset(std::max<size_t>(1, nh1) * sizeof(cx<T>));
+        hist[1].reset(std::max<size_t>(1, nh1) * sizeof(cx<T>));
+        hip_check(hipStreamSynchronize(s), "filter spectrum");
+        reset();
+    }
+    void reset()
+    {
+        hipStream_t s = library_stream();
+        for (auto& h : hist)
+        {
+            const long long cnt = (long long)(h.bytes / sizeof(cx<T>));
+            hipLaunchKernelGGL(k_zero<cx<T>>, dim3(nblocks(cnt, 256)), dim3(256), 0, s, h.template as<cx<T>>(), cnt);
+            hip_check(hipGetLastError(), "filter reset");
+        }
+        hip_check(hipStreamSynchronize(s), "filter reset");
+        cur = 0;
+    }
+    void process(cx<T>* out, const cx<T>* in, size_t n, hipStream_t s)
+    {
+        if (n == 0)
+            return;
+        const long long nseg = (long long)((n + L - 1) / L);
+        seg.ensure((size_t)nseg * S * sizeof(cx<T>));
+        yseg.ensure((size_t)nseg * S * sizeof(cx<T>));
+        const cx<T>* h0 = hist[cur].template as<cx<T>>();
+        hipLaunchKernelGGL(k_gather_segments<cx<T>>, dim3(nblocks(nseg * (long long)S, 256)), dim3(256), 0, s,
+                           seg.template as<cx<T>>(), h0, in, (long long)nh1, (long long)n, (long long)S, (long long)L,
+                           nseg);
+        hip_check(hipGetLastError(), "filter gather");
+        f->run(load_plain<T>(seg.ptr, (long long)S), store_plain<T>(seg.ptr, (long long)S), nseg, false, s);
+        xload<T> lm = load_plain<T>(seg.ptr, (long long)S);
+        lm.mode     = ld_mul;
+        lm.q        = H.template as<cx<T>>();
+        f->run(lm, store_plain<T>(yseg.ptr, (long long)S), nseg, true, s);
+        hipLaunchKernelGGL(k_scatter_valid<cx<T>>, dim3(nblocks(nseg * (long long)L, 256)), dim3(256), 0, s, out,
+                           yseg.template as<cx<T>>(), (long long)nh1, (long long)n, (long long)S, (long long)L, nseg);
+        hip_check(hipGetLastError(), "filter scatter");
+        if (nh1 > 0)
+        {
+            hipLaunchKernelGGL(k_update_history<cx<T>>, dim3(nblocks((long long)nh1, 256)), dim3(256), 0, s,
+                               hist[1 - cur].template as<cx<T>>(), h0, in, (long long)nh1, (long long)n);
+            hip_check(hipGetLastError(), "filter history");
+            cur = 1 - cur;
+        }
+    }
+};
+
+template <typename T>
+gpu_conv_filter_complex<T>::gpu_conv_filter_complex(const void* taps_host, size_t ntaps, size_t block_size)
+    : p(new impl(taps_host, ntaps, block_size))
+{
+}
+template <typename T>
+gpu_conv_filter_complex<T>::~gpu_conv_filter_complex() = default;
+template <typename T>
+size_t gpu_conv_filter_complex<T>::block_size() const
+{
+    return p->block;
+}
+template <typename T>
+void gpu_conv_filter_complex<T>::set_data(const void* taps_host, size_t ntaps)
+{
+    p->set(taps_host, ntaps);
+}
+template <typename T>
+void gpu_conv_filter_complex<T>::reset()
+{
+    p->reset();
+}
+template <typename T>
+void gpu_conv_filter_complex<T>::process(void* out, const void* in, size_t n, hipStream_t s)
+{
+    p->process(static_cast<cx<T>*>(out), static_cast<const cx<T>*>(in), n, s);
+}
+
 template <typename T>
 gpu_conv_filter<T>::gpu_conv_filter(const T* taps_host, size_t ntaps, size_t block_size)
     : p(new impl(taps_host, ntaps, block_size))
@@ -1295,9 +1493,12 @@ template class gpu_conv<float>;
 template class gpu_conv<double>;
 template class gpu_conv_filter<float>;
 template class gpu_conv_filter<double>;
-template void gpu_convolve_complex<float>(void*, const void*, size_t, const void*, size_t, bool, bool, hipStream_t);
+template class gpu_conv_filter_complex<float>;
+template class gpu_conv_filter_complex<double>;
+template void gpu_convolve_complex<float>(void*, const void*, size_t, const void*, size_t, bool, bool, gpu_dft<float>&,
+                                          void*, void*, hipStream_t);
 template void gpu_convolve_complex<double>(void*, const void*, size_t, const void*, size_t, bool, bool,
-                                           hipStream_t);
+                                           gpu_dft<double>&, void*, void*, hipStream_t);
 template void gpu_scale<float>(void*, size_t, float, hipStream_t);
 template void gpu_scale<double>(void*, size_t, double, hipStream_t);
 

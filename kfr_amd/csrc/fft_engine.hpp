@@ -75,6 +75,15 @@ public:
     // general strided batches: element i of transform b at base + b*dist + i (complex units)
     void execute_dist(void* out, long long odist, const void* in, long long idist, long long batch,
                       bool inverse, hipStream_t s);
+    // Progressive execution of one transform in place on device data (n values): progressive_prepare
+    // lowers the transform to about `want` units (one launch each; Stockham passes of a finer schedule
+    // for lengths in one workgroup, the column and the row kernel of a four-step, the whole transform
+    // for Bluestein) and returns their count; progressive_unit(0 .. count-1) then run in order.
+    // Four-step lengths keep their intermediate in `mid` (progressive_mid_bytes()), apart from the
+    // plan's own scratch, which other executions may use between units.
+    int progressive_prepare(int want);
+    size_t progressive_mid_bytes() const;
+    void progressive_unit(int u, void* data, void* mid, bool inverse, hipStream_t s);
     struct impl;
     std::unique_ptr<impl> p;
 };
@@ -92,6 +101,13 @@ public:
     const schedule_info& schedule() const;
     void forward(void* cout, const void* rin, long long batch, hipStream_t s);
     void inverse(void* rout, const void* cin, long long batch, hipStream_t s);
+    // Progressive execution of one transform: the half-length complex units plus one unit that repacks
+    // (forward, the last unit) or unpacks (inverse, the first unit) the spectrum. z holds n/2 complex
+    // values: the input reals (forward) or the output reals (inverse); c holds complex_size() values:
+    // the output spectrum (forward) or the input spectrum (inverse).
+    int progressive_prepare(int want);
+    size_t progressive_mid_bytes() const;
+    void progressive_unit(int u, void* z, void* c, void* mid, bool inverse, hipStream_t s);
     struct impl;
     std::unique_ptr<impl> p;
 };
@@ -133,14 +149,31 @@ public:
     std::unique_ptr<impl> p;
 };
 
-// Complex convolution (convolve of complex<T> vectors): complex FFT of next_pow2(n1+n2-1) points.
-// keep_imag = false reproduces the reference, which returns only the real part.
+// Complex convolution (convolve of complex<T> vectors): complex FFT of S = next_pow2(na+nb-1) points
+// with the plan f (of S points) through the device buffers A and B (S values each). keep_imag = false
+// reproduces the reference, which returns only the real part.
 template <typename T>
 void gpu_convolve_complex(void* y, const void* a, size_t na, const void* b, size_t nb, bool correlate,
-                          bool keep_imag, hipStream_t s);
+                          bool keep_imag, gpu_dft<T>& f, void* A, void* B, hipStream_t s);
 
-// Streaming uniformly-partitioned overlap-add convolution filter (convolve_filter<T>), state on the
-// device. Real T only.
+// Streaming convolution filter of complex signals with complex taps (convolve_filter<complex<T>>),
+// state on the device: overlap-save segments through complex FFTs.
+template <typename T>
+class gpu_conv_filter_complex
+{
+public:
+    gpu_conv_filter_complex(const void* taps_host, size_t ntaps, size_t block_size);
+    ~gpu_conv_filter_complex();
+    size_t block_size() const;
+    void set_data(const void* taps_host, size_t ntaps);
+    void reset();
+    // process n complex samples (device pointers), any n; keeps state across calls
+    void process(void* out, const void* in, size_t n, hipStream_t s);
+    struct impl;
+    std::unique_ptr<impl> p;
+};
+
+// Streaming convolution filter (convolve_filter<T>), state on the device. Real T (T taps, T signal).
 template <typename T>
 class gpu_conv_filter
 {

@@ -59,14 +59,38 @@ def _stream_ptr(stream):
     return h or None
 
 
+class progressive:
+    """State of a progressive execution (dft_plan::progressive, include/kfr/dft/fft.hpp:364-374): a native
+    handle that keeps the plan and the buffers alive until the last step."""
+
+    def __init__(self, handle, keep):
+        self._h = handle
+        self._keep = keep
+        self.step = 0
+        self.total = int(lib().kfr_amd_progressive_total_steps(handle))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().kfr_amd_progressive_delete(h)
+            self._h = None
+
+
+def _progressive_step(prog: progressive) -> bool:
+    r = lib().kfr_amd_progressive_step(prog._h)
+    check(r >= 0)
+    prog.step += 1
+    return r == 1
+
+
 class dft_plan:
     """Complex DFT of `size` points (include/kfr/dft/fft.hpp:183-456)."""
 
-    def __init__(self, size: int, dtype=np.complex64, progressive_optimized: bool = False):
+    def __init__(self, size: int, dtype=np.complex64, progressive_optimized: bool = False, _handle=None):
         self.t = _tag(dtype)
         self._progressive_optimized = progressive_optimized
         L = lib()
-        self._h = getattr(L, f"kfr_dft_create_plan_{self.t}")(size)
+        self._h = _handle if _handle is not None else getattr(L, f"kfr_dft_create_plan_{self.t}")(size)
         if not self._h:
             raise KfrError(L.kfr_last_error().decode())
         self.size = getattr(L, f"kfr_dft_get_size_{self.t}")(self._h)
@@ -96,21 +120,21 @@ class dft_plan:
         return describe(3 if self._progressive_optimized else 0, self.t, self.size)
 
     def progressive_total_steps(self) -> int:
-        return lib().kfr_amd_dft_progressive_total_steps(0 if self.t == "f32" else 1, self.size)
+        return lib().kfr_amd_dft_progressive_steps(0 if self.t == "f32" else 1, self.size,
+                                                    int(self._progressive_optimized))
 
-    # progressive execution (include/kfr/dft/fft.hpp:393-439): the result is complete in `out` after
-    # the last step; earlier steps only advance the counter.
-    def progressive_start(self, inverse, out, in_, temp=None):
+    # progressive execution (include/kfr/dft/fft.hpp:393-439): each step runs its share of the
+    # transform's GPU launches synchronously; `out` holds the result after the last step
+    def progressive_start(self, inverse, out, in_, temp=None) -> progressive:
         if self.temp_size and temp is None:
             raise KfrError("Temporary buffer must be provided for progressive execution")
-        return {"inverse": bool(inverse), "out": out, "in": in_, "temp": temp, "step": 0}
+        h = getattr(lib(), f"kfr_amd_dft_progressive_start_{self.t}")(self._h, int(self._progressive_optimized),
+                                                                      int(bool(inverse)), _ptr(out), _ptr(in_))
+        check(bool(h))
+        return progressive(h, (self, out, in_))
 
-    def progressive_step(self, prog) -> bool:
-        total = self.progressive_total_steps()
-        if prog["step"] == total - 1:
-            self.execute(prog["out"], prog["in"], prog["temp"], prog["inverse"])
-        prog["step"] += 1
-        return prog["step"] < total
+    def progressive_step(self, prog: progressive) -> bool:
+        return _progressive_step(prog)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -122,10 +146,10 @@ class dft_plan:
 class dft_plan_real:
     """Real DFT of `size` (even) points (include/kfr/dft/fft.hpp:459-566)."""
 
-    def __init__(self, size: int, fmt: int = CCs, dtype=np.float32):
+    def __init__(self, size: int, fmt: int = CCs, dtype=np.float32, _handle=None):
         self.t = _tag(dtype)
         L = lib()
-        self._h = getattr(L, f"kfr_dft_real_create_plan_{self.t}")(size, fmt)
+        self._h = _handle if _handle is not None else getattr(L, f"kfr_dft_real_create_plan_{self.t}")(size, fmt)
         if not self._h:
             raise KfrError(L.kfr_last_error().decode())
         self.size = getattr(L, f"kfr_dft_real_get_size_{self.t}")(self._h)
@@ -148,6 +172,22 @@ class dft_plan_real:
                                                                         int(inverse), _stream_ptr(stream))
         check(r == 0)
         return out
+
+    def progressive_total_steps(self) -> int:
+        return lib().kfr_amd_dft_real_progressive_steps(0 if self.t == "f32" else 1, self.size, self.fmt)
+
+    def progressive_start(self, out, in_, temp=None) -> progressive:
+        """Forward when `in_` is real, inverse when it is complex (include/kfr/dft/fft.hpp:538-556)."""
+        if self.temp_size and temp is None:
+            raise KfrError("Temporary buffer must be provided for progressive execution")
+        is_complex = np.dtype(in_.dtype).kind == "c" if isinstance(in_, np.ndarray) else in_.is_complex()
+        h = getattr(lib(), f"kfr_amd_dft_real_progressive_start_{self.t}")(self._h, int(is_complex), _ptr(out),
+                                                                           _ptr(in_))
+        check(bool(h))
+        return progressive(h, (self, out, in_))
+
+    def progressive_step(self, prog: progressive) -> bool:
+        return _progressive_step(prog)
 
     def dump(self):
         getattr(lib(), f"kfr_dft_real_dump_{self.t}")(self._h)
@@ -304,15 +344,23 @@ def autocorrelate(a):
 
 
 class convolve_filter:
-    """Streaming FFT convolution filter (include/kfr/dft/convolution.hpp:99-148), real types."""
+    """Streaming FFT convolution filter (include/kfr/dft/convolution.hpp:99-148), real or complex taps
+    (complex through the kfr_amd_filter_*_c32/c64 extensions)."""
 
     def __init__(self, taps, block_size: int = 1024):
         taps = np.ascontiguousarray(taps)
         self.t = _tag(taps.dtype)
+        if taps.dtype.kind == "c":
+            self.t = "c32" if self.t == "f32" else "c64"
         self.dtype = taps.dtype
         L = lib()
-        self._h = getattr(L, f"kfr_filter_create_convolution_plan_{self.t}")(taps.ctypes.data, taps.size,
-                                                                             block_size)
+        self._create = (f"kfr_amd_filter_create_convolution_plan_{self.t}" if self.t[0] == "c"
+                        else f"kfr_filter_create_convolution_plan_{self.t}")
+        self._process = f"kfr_amd_filter_process_{self.t}" if self.t[0] == "c" else f"kfr_filter_process_{self.t}"
+        self._reset = f"kfr_amd_filter_reset_{self.t}" if self.t[0] == "c" else f"kfr_filter_reset_{self.t}"
+        self._delete = (f"kfr_amd_filter_delete_plan_{self.t}" if self.t[0] == "c"
+                        else f"kfr_filter_delete_plan_{self.t}")
+        self._h = getattr(L, self._create)(taps.ctypes.data, taps.size, block_size)
         if not self._h:
             raise KfrError(L.kfr_last_error().decode())
         b = 1
@@ -328,27 +376,24 @@ class convolve_filter:
             src = dest
         n = min(dest.size if isinstance(dest, np.ndarray) else dest.numel(),
                 src.size if isinstance(src, np.ndarray) else src.numel())
-        getattr(lib(), f"kfr_filter_process_{self.t}")(self._h, _ptr(dest), _ptr(src), n)
+        getattr(lib(), self._process)(self._h, _ptr(dest), _ptr(src), n)
         check()
         return dest
 
     def reset(self):
-        getattr(lib(), f"kfr_filter_reset_{self.t}")(self._h)
+        getattr(lib(), self._reset)(self._h)
         check()
 
     def set_data(self, taps):
-        """Replace the filter taps (and reset the state)."""
+        """Replace the filter taps and reset the state (src/dft/convolution-impl.cpp:58-84)."""
         taps = np.ascontiguousarray(taps, dtype=self.dtype)
-        getattr(lib(), f"kfr_filter_delete_plan_{self.t}")(self._h)
-        self._h = getattr(lib(), f"kfr_filter_create_convolution_plan_{self.t}")(taps.ctypes.data, taps.size,
-                                                                                 self.block_size)
-        if not self._h:
-            raise KfrError(lib().kfr_last_error().decode())
+        getattr(lib(), f"kfr_amd_filter_set_data_{self.t}")(self._h, taps.ctypes.data, taps.size)
+        check()
 
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            getattr(lib(), f"kfr_filter_delete_plan_{self.t}")(h)
+            getattr(lib(), self._delete)(h)
             self._h = None
 
 
@@ -428,14 +473,12 @@ class dft_plan_half:
 
 
 class dft_cache:
-    """Process-wide cache of complex plans (include/kfr/dft/cache.hpp:44-126). The key includes the
-    element type and the size, as the reference's does."""
+    """The process-wide plan cache (include/kfr/dft/cache.hpp:44-126) held by the library
+    (kfr_amd_cache_*): get() and getreal() return plans that share the cached GPU plan; its key names
+    the element type, transform kind, size, pack format, batch layout and schedule variant."""
 
     _inst = None
     _lock = threading.Lock()
-
-    def __init__(self):
-        self._plans = {}
 
     @classmethod
     def instance(cls) -> "dft_cache":
@@ -445,33 +488,58 @@ class dft_cache:
             return cls._inst
 
     def get(self, dtype, size: int) -> dft_plan:
-        key = (_tag(dtype), int(size))
-        with self._lock:
-            p = self._plans.get(key)
-            if p is None:
-                p = dft_plan(size, np.complex64 if key[0] == "f32" else np.complex128)
-                self._plans[key] = p
-            return p
+        t = _tag(dtype)
+        h = getattr(lib(), f"kfr_amd_cache_dft_plan_{t}")(int(size))
+        check(bool(h))
+        return dft_plan(size, np.complex64 if t == "f32" else np.complex128, _handle=h)
+
+    def getreal(self, dtype, size: int, fmt: int = CCs) -> dft_plan_real:
+        t = _tag(dtype)
+        h = getattr(lib(), f"kfr_amd_cache_dft_real_plan_{t}")(int(size), fmt)
+        check(bool(h))
+        return dft_plan_real(size, fmt, np.float32 if t == "f32" else np.float64, _handle=h)
+
+    def size(self) -> int:
+        return int(lib().kfr_amd_cache_size())
 
     def clear(self):
-        with self._lock:
-            self._plans.clear()
+        lib().kfr_amd_cache_clear()
+        check()
 
 
 def dft(x: np.ndarray) -> np.ndarray:
-    """Forward DFT through the cache (include/kfr/dft/cache.hpp:128-139)."""
+    """Forward DFT through the cache (include/kfr/dft/cache.hpp:128-137)."""
     x = np.ascontiguousarray(x)
     p = dft_cache.instance().get(x.dtype, x.size)
-    out = np.empty_like(x)
+    out = np.full_like(x, np.nan)
     return p.execute(out, x)
 
 
 def idft(x: np.ndarray) -> np.ndarray:
-    """Inverse DFT through the cache, unscaled (include/kfr/dft/cache.hpp:140-151)."""
+    """Inverse DFT through the cache, unscaled (include/kfr/dft/cache.hpp:139-148)."""
     x = np.ascontiguousarray(x)
     p = dft_cache.instance().get(x.dtype, x.size)
-    out = np.empty_like(x)
+    out = np.full_like(x, np.nan)
     return p.execute(out, x, inverse=True)
+
+
+def realdft(x: np.ndarray) -> np.ndarray:
+    """Real forward DFT through the cache, n/2+1 values in CCs format (include/kfr/dft/cache.hpp:150-159)."""
+    x = np.ascontiguousarray(x)
+    p = dft_cache.instance().getreal(x.dtype, x.size)
+    out = np.full(x.size // 2 + 1, np.nan, dtype=np.complex64 if x.dtype == np.float32 else np.complex128)
+    return p.execute(out, x)
+
+
+def irealdft(X: np.ndarray) -> np.ndarray:
+    """Real inverse DFT through the cache of a CCs spectrum of (n-1)*2 points, unscaled
+    (include/kfr/dft/cache.hpp:161-170)."""
+    X = np.ascontiguousarray(X)
+    n = (X.size - 1) * 2
+    rdt = np.float32 if X.dtype == np.complex64 else np.float64
+    p = dft_cache.instance().getreal(rdt, n)
+    out = np.full(n, np.nan, dtype=rdt)
+    return p.execute(out, X)
 
 
 def describe(kind: int, t: str, size: int, arg: int = 0) -> str:

@@ -2,6 +2,7 @@
 //   emu_main dft  f32|f64 n batch inverse in.bin out.bin
 //   emu_main real f32|f64 n fmt batch inverse in.bin out.bin
 //   emu_main dct  f32|f64 n batch inverse in.bin out.bin
+//   emu_main prog f32|f64 n want inverse in.bin out.bin   (progressive units of one transform)
 // The input is copied into a buffer of exactly its size and the output buffer holds exactly the
 // output, so under AddressSanitizer every access beyond either is reported.
 #include "../../kfr_amd/csrc/fft_engine.hpp"
@@ -33,6 +34,38 @@ static int run(const std::string& kind, int argc, char** argv)
 {
     const size_t n = std::strtoull(argv[3], nullptr, 10);
     int a          = 4;
+    if (kind == "prog")
+    {
+        const int want     = std::atoi(argv[4]);
+        const bool inverse = std::atoi(argv[5]) != 0;
+        const std::vector<char> in = read_file(argv[6]);
+        const size_t bytes         = n * 2 * sizeof(T);
+        if (in.size() != bytes)
+            return 2;
+        char* d = static_cast<char*>(std::malloc(bytes));
+        std::memcpy(d, in.data(), bytes);
+        try
+        {
+            gpu_dft<T> f(n);
+            const int units = f.progressive_prepare(want);
+            void* mid       = f.progressive_mid_bytes() ? std::malloc(f.progressive_mid_bytes()) : nullptr;
+            for (int u = 0; u < units; ++u)
+                f.progressive_unit(u, d, mid, inverse, nullptr);
+            std::free(mid);
+            std::fprintf(stderr, "units=%d\n", units);
+        }
+        catch (const std::exception& e)
+        {
+            std::fprintf(stderr, "error: %s\n", e.what());
+            return 3;
+        }
+        FILE* f = std::fopen(argv[7], "wb");
+        if (!f || std::fwrite(d, 1, bytes, f) != bytes)
+            return 4;
+        std::fclose(f);
+        std::free(d);
+        return 0;
+    }
     int fmt        = 1;
     if (kind == "real")
         fmt = std::atoi(argv[a++]);
@@ -94,7 +127,7 @@ int main(int argc, char** argv)
         return 2;
     }
     const std::string kind = argv[1], t = argv[2];
-    if (kind != "dft" && kind != "real" && kind != "dct")
+    if (kind != "dft" && kind != "real" && kind != "dct" && kind != "prog")
         return 2;
     return t == "f64" ? run<double>(kind, argc, argv) : run<float>(kind, argc, argv);
 }
