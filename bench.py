@@ -581,8 +581,14 @@ def main() -> int:
 
 
 def cpu_baseline(cfgs):
-    """The reference engine (oracle/_ref, built from the reference sources) on the host cores, on a
-    bounded sample of the complex-f32 sweep: each size runs a reduced batch, split across threads."""
+    """The reference engine (oracle/_ref, built from the reference sources) on the host cores, timed as
+    the reference's benchmarking guidance asks: one shared plan per point, the batch split across the
+    threads (each with its own temp buffer), planning excluded, at least 1 s per point as the median of
+    5 repetitions of at least 0.2 s. The sample is 10 points of the sweep (complex f32 and f64 at short,
+    medium and long lengths, R2C f32 at 2^24, DCT-II f32 at 65536); every configuration of the sweep is
+    charged at the rate of the sampled point of its kind and precision nearest in log2 N, and the value
+    is the sweep's flops over the time that gives, the weighting of the GPU figure. Threads: the box's
+    CPU share (16 for one GPU) or the cores this process may use, whichever is smaller."""
     try:
         import oracle
     except ImportError:
@@ -591,29 +597,55 @@ def cpu_baseline(cfgs):
     if ref is None:
         return {"value": None, "unit": "GFLOP/s", "cores": 0, "kind": "reference",
                 "sample": "skipped: oracle/_ref not built"}
-    threads = max(1, min(16, os.cpu_count() or 1))
-    tot_f = 0.0
-    tot_t = 0.0
-    sizes = []
-    c2c = [c for c in cfgs if c[0] == "c2c" and c[1] == "f32"]
-    pts = max(max(min(c[3], max(1, (1 << 21) // c[2])) * c[2] for c in c2c), 1)
-    pool = oracle.complex_gaussian(11, pts, np.complex64)
-    used = 0
-    for kind, dt, n, batch in c2c:
-        b = min(batch, max(1, (1 << 21) // n))
-        x = pool[: b * n].reshape(b, n)
-        out = np.empty_like(x)
-        th = min(threads, b)
-        t = ref.time(0, 0, n, b, False, th, 2, out, x)
-        tot_f += flops("c2c", n, b)
-        tot_t += t
-        used = max(used, th)
-        sizes.append(f"{n}x{b}")
-    return {"value": round(tot_f / tot_t * 1e-9, 2), "unit": "GFLOP/s", "cores": used, "kind": "reference",
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    threads = max(1, min(16, avail))
+    sample = [("c2c", "f32", 1 << 10), ("c2c", "f32", 1 << 14), ("c2c", "f32", 1 << 18), ("c2c", "f32", 1 << 22),
+              ("c2c", "f32", 1 << 24), ("c2c", "f64", 1 << 10), ("c2c", "f64", 1 << 18), ("c2c", "f64", 1 << 24),
+              ("r2c", "f32", 1 << 24), ("dct", "f32", 65536)]
+    kind_code = {"c2c": 0, "r2c": 1, "dct": 2}
+    rates = {}
+    points = []
+    rng = np.random.default_rng(11)
+    for kind, dt, n in sample:
+        real = kind != "c2c"
+        fdt = np.float32 if dt == "f32" else np.float64
+        cdt = np.complex64 if dt == "f32" else np.complex128
+        es = np.dtype(fdt if real else cdt).itemsize
+        cap = max(1, (1 << 30) // (n * es))                   # at most 1 GiB of input
+
+        def run(batch, reps):
+            x = rng.standard_normal((batch, n)).astype(fdt) if real else \
+                (rng.standard_normal((batch, n)) + 1j * rng.standard_normal((batch, n))).astype(cdt)
+            out = np.empty((batch, n // 2 + 1), cdt) if kind == "r2c" else np.empty_like(x)
+            return ref.time(kind_code[kind], 0 if dt == "f32" else 1, n, batch, False, min(threads, batch), reps,
+                            out, x)
+
+        b = min(cap, threads)
+        t = run(b, 1)
+        while t < 0.2 and b < cap:                             # grow the batch to >= 0.2 s per repetition
+            b = min(cap, max(b + 1, int(b * 0.25 / max(t, 1e-6))))
+            t = run(b, 1)
+        t = run(b, 5)
+        r = flops(kind, n, b) / t
+        rates[(kind, dt, n)] = r
+        points.append(f"{kind} {dt} N=2^{int(math.log2(n))} x{b}: {r * 1e-9:.2f}")
+
+    def rate_for(kind, dt, n):
+        cands = [(abs(math.log2(k[2]) - math.log2(n)), v) for k, v in rates.items() if k[0] == kind and k[1] == dt]
+        if not cands:
+            cands = [(abs(math.log2(k[2]) - math.log2(n)), v) for k, v in rates.items() if k[0] == "c2c" and k[1] == dt]
+        return min(cands)[1]
+
+    tot_f = sum(flops(k, n, b) for k, dt, n, b in cfgs)
+    tot_t = sum(flops(k, n, b) / rate_for(k, dt, n) for k, dt, n, b in cfgs)
+    return {"value": round(tot_f / tot_t * 1e-9, 2), "unit": "GFLOP/s", "cores": threads, "kind": "reference",
             "arch": ref.version(),
-            "sample": "complex-f32 forward, N=2^4..2^24, 2^21 points per size (one transform at N>2^21): " +
-                      ", ".join(sizes) + "; median of 2 runs per size, batch split over up to "
-                      f"{threads} threads"}
+            "sample": "reference dft_plan / dft_plan_real / dct_plan on " + str(threads) + " threads, median of 5 "
+                      "repetitions of >= 0.2 s per point, GFLOP/s per point: " + "; ".join(points) +
+                      "; the sweep's configurations charged at the nearest sampled rate"}
 
 
 if __name__ == "__main__":

@@ -948,7 +948,7 @@ struct core_fft
     }
 
     template <bool INV>
-    void run(const xload<T>& ld, const xstore<T>& st, long long batch, hipStream_t s)
+    void run(const xload<T>& ld, const xstore<T>& st, long long batch, hipStream_t s, workspace w = {})
     {
         if (batch <= 0)
             return;
@@ -957,9 +957,9 @@ struct core_fft
             const cx<T>* x = static_cast<const cx<T>*>(ld.p) + ld.b0 * (long long)n;
             cx<T>* y       = static_cast<cx<T>*>(st.p) + st.b0 * (long long)n;
             if (pow2)
-                pow2->run(INV, x, y, batch, s);
+                pow2->run(INV, x, y, batch, s, w);
             else
-                mixed->run(INV, x, y, batch, s);
+                mixed->run(INV, x, y, batch, s, w);
             return;
         }
         const cx<T>* t = tw.as<cx<T>>();
@@ -973,6 +973,12 @@ struct core_fft
                                             (unsigned)(S.tpt * tpb), sh, s);
             return;
         }
+        if (w.p)
+        {
+            run_fourstep<INV>(ld, st, batch, static_cast<cx<T>*>(w.p), 3, s,
+                              rounds_in(w, chunk, n * sizeof(cx<T>), 1));
+            return;
+        }
         scratch.ensure((size_t)std::min<long long>(chunk, batch) * n * sizeof(cx<T>));
         run_fourstep<INV>(ld, st, batch, scratch.as<cx<T>>(), 3, s);
     }
@@ -981,8 +987,9 @@ struct core_fft
     // bit 1 the row kernels (one of them alone only within one round)
     template <bool INV>
     void run_fourstep(const xload<T>& ld, const xstore<T>& st, long long batch, cx<T>* mid, int phases,
-                      hipStream_t s)
+                      hipStream_t s, long long round = 0)
     {
+        const long long chunk = round > 0 ? round : this->chunk;
         const cx<T>* t   = tw.as<cx<T>>();
         lds_geom GA      = make_geom(SA, C, 1, C, 1);
         lds_geom GB      = make_geom(SB, R, 0, 1, (int)n1 + 1);

@@ -262,15 +262,17 @@ struct gpu_dft<T>::impl
     }
 
     template <bool INV>
-    void run_t(const xload<T>& ld, const xstore<T>& st, long long batch, hipStream_t s)
+    void run_t(const xload<T>& ld, const xstore<T>& st, long long batch, hipStream_t s, workspace w = {})
     {
         if (batch <= 0)
             return;
         if (algo == CORE)
         {
-            core->template run<INV>(ld, st, batch, s);
+            core->template run<INV>(ld, st, batch, s, w);
             return;
         }
+        if (w.p && algo == BS_CHAIN)
+            throw std::logic_error("this Bluestein length keeps intermediates of its own and cannot run in a workspace");
         // the chirp post-multiplies the input and pre-multiplies the output (conjugated for the inverse)
         xload<T> lc  = ld;
         lc.mtab      = chirp.as<cx<T>>();
@@ -342,12 +344,13 @@ struct gpu_dft<T>::impl
         }
     }
 
-    void run(const xload<T>& ld, const xstore<T>& st, long long batch, bool inverse, hipStream_t s)
+    void run(const xload<T>& ld, const xstore<T>& st, long long batch, bool inverse, hipStream_t s,
+             workspace w = {})
     {
         if (inverse)
-            run_t<true>(ld, st, batch, s);
+            run_t<true>(ld, st, batch, s, w);
         else
-            run_t<false>(ld, st, batch, s);
+            run_t<false>(ld, st, batch, s, w);
     }
 
     // progressive execution (see gpu_dft::progressive_prepare)
@@ -406,6 +409,13 @@ template <typename T>
 void gpu_dft<T>::execute(void* out, const void* in, long long batch, bool inverse, hipStream_t s)
 {
     execute_dist(out, (long long)p->n, in, (long long)p->n, batch, inverse, s);
+}
+
+template <typename T>
+void gpu_dft<T>::execute_workspace(void* out, const void* in, long long batch, bool inverse, hipStream_t s,
+                                  workspace w)
+{
+    p->run(load_plain<T>(in, (long long)p->n), store_plain<T>(out, (long long)p->n), batch, inverse, s, w);
 }
 
 template <typename T>

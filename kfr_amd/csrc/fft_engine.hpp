@@ -7,6 +7,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <algorithm>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -25,6 +26,35 @@ void hip_check(hipError_t e, const char* what);
 // Test hook: when > 0, the n-th following device allocation made by the library fails (counted down
 // on every allocation), so the error path of every entry point can be exercised from the host.
 void set_alloc_failure_countdown(int n);
+
+// Caller-provided device memory for a plan's intermediates (four-step and multi-pass rounds): when p
+// is set, the plan uses it instead of allocating its own scratch and runs in rounds that fit `bytes`
+// (rounds_in); a plan that needs intermediates beyond one item's worth there throws.
+struct workspace
+{
+    void* p      = nullptr;
+    size_t bytes = 0;
+};
+
+// batch items per round: `chunk`, or what `buffers` equal 256-byte aligned parts of the workspace hold
+// at `per_item` bytes each
+inline long long rounds_in(const workspace& w, long long chunk, size_t per_item, int buffers)
+{
+    if (!w.p)
+        return chunk;
+    const size_t each = (w.bytes / (size_t)buffers) & ~(size_t)255;
+    const long long c = (long long)(each / per_item);
+    if (c < 1)
+        throw std::length_error("the workspace of " + std::to_string(w.bytes) + " bytes cannot hold the " +
+                                std::to_string((size_t)buffers * per_item) + " bytes of one item's intermediate");
+    return std::min(chunk, c);
+}
+// the b-th of `buffers` equal aligned parts of the workspace
+inline void* workspace_part(const workspace& w, int b, int buffers)
+{
+    const size_t each = (w.bytes / (size_t)buffers) & ~(size_t)255;
+    return static_cast<char*>(w.p) + (size_t)b * each;
+}
 
 // Owned device allocation
 struct dev_mem
@@ -75,6 +105,9 @@ public:
     // general strided batches: element i of transform b at base + b*dist + i (complex units)
     void execute_dist(void* out, long long odist, const void* in, long long idist, long long batch,
                       bool inverse, hipStream_t s);
+    // execute() with the intermediates in `w` (no allocation by the plan); Bluestein lengths whose
+    // transform does not fit one workgroup are refused
+    void execute_workspace(void* out, const void* in, long long batch, bool inverse, hipStream_t s, workspace w);
     // Progressive execution of one transform in place on device data (n values): progressive_prepare
     // lowers the transform to about `want` units (one launch each; Stockham passes of a finer schedule
     // for lengths in one workgroup, the column and the row kernel of a four-step, the whole transform

@@ -544,7 +544,7 @@ template <typename T>
 mixed_fft<T>::~mixed_fft() = default;
 
 template <typename T>
-void mixed_fft<T>::run(bool inverse, const void* in, void* out, long long batch, hipStream_t s)
+void mixed_fft<T>::run(bool inverse, const void* in, void* out, long long batch, hipStream_t s, workspace w)
 {
     if (batch <= 0)
         return;
@@ -562,11 +562,18 @@ void mixed_fft<T>::run(bool inverse, const void* in, void* out, long long batch,
     }
     const long long n  = (long long)p->n;
     const long long n1 = mx::menu[p->i1], n2 = mx::menu[p->i2];
-    p->scratch.ensure((size_t)std::min(p->chunk, batch) * n * sizeof(cx<float>));
-    cx<float>* z = p->scratch.template as<cx<float>>();
-    for (long long b0 = 0; b0 < batch; b0 += p->chunk)
+    const long long chunk = rounds_in(w, p->chunk, (size_t)n * sizeof(cx<float>), 1);
+    cx<float>* z;
+    if (w.p)
+        z = static_cast<cx<float>*>(w.p);
+    else
     {
-        const long long nb = std::min(p->chunk, batch - b0);
+        p->scratch.ensure((size_t)std::min(chunk, batch) * n * sizeof(cx<float>));
+        z = p->scratch.template as<cx<float>>();
+    }
+    for (long long b0 = 0; b0 < batch; b0 += chunk)
+    {
+        const long long nb = std::min(chunk, batch - b0);
         mx::margs a;
         a.ta    = t + p->o_ta;
         a.tb    = t + p->o_tb;

@@ -21,7 +21,7 @@ public:
     mixed_fft(size_t n, hipStream_t s);
     ~mixed_fft();
     // batch transforms of contiguous items (distance n), in == out allowed, unscaled
-    void run(bool inverse, const void* in, void* out, long long batch, hipStream_t s);
+    void run(bool inverse, const void* in, void* out, long long batch, hipStream_t s, workspace w = {});
     std::string describe() const;
 
 private:

@@ -1176,7 +1176,7 @@ struct pow2_fft<T>::impl
     dev_mem scratch2;
 
     template <int PREC>
-    void run(bool inverse, const void* in, void* out, long long batch, hipStream_t s);
+    void run(bool inverse, const void* in, void* out, long long batch, hipStream_t s, workspace w);
 };
 
 template <typename T>
@@ -1307,7 +1307,7 @@ pow2_fft<T>::~pow2_fft() = default;
 
 template <typename T>
 template <int PREC>
-void pow2_fft<T>::impl::run(bool inverse, const void* in, void* out, long long batch, hipStream_t s)
+void pow2_fft<T>::impl::run(bool inverse, const void* in, void* out, long long batch, hipStream_t s, workspace w)
 {
     using p2::args;
     constexpr int ML = p2::max_log<T>;
@@ -1317,12 +1317,24 @@ void pow2_fft<T>::impl::run(bool inverse, const void* in, void* out, long long b
     {
         if (multi)
         {
-            const long long n = 1ll << L;
-            const size_t bytes = (size_t)std::min(chunk, batch) * n * sizeof(cx<T>);
-            scratch.ensure(bytes);
-            if (mp_log.size() > 2)
-                scratch2.ensure(bytes);
-            void* buf[2] = { scratch.ptr, scratch2.ptr };
+            const long long n     = 1ll << L;
+            const int nbuf        = mp_log.size() > 2 ? 2 : 1;
+            const long long chunk = rounds_in(w, this->chunk, (size_t)n * sizeof(cx<T>), nbuf);
+            void* buf[2]          = { nullptr, nullptr };
+            if (w.p)
+            {
+                buf[0] = workspace_part(w, 0, nbuf);
+                buf[1] = nbuf > 1 ? workspace_part(w, 1, nbuf) : nullptr;
+            }
+            else
+            {
+                const size_t bytes = (size_t)std::min(chunk, batch) * n * sizeof(cx<T>);
+                scratch.ensure(bytes);
+                if (nbuf > 1)
+                    scratch2.ensure(bytes);
+                buf[0] = scratch.ptr;
+                buf[1] = scratch2.ptr;
+            }
             const int np = (int)mp_log.size();
             for (long long b0 = 0; b0 < batch; b0 += chunk)
             {
@@ -1359,7 +1371,15 @@ void pow2_fft<T>::impl::run(bool inverse, const void* in, void* out, long long b
     }
     const long long n = 1ll << L;
     const size_t mid  = PREC == p2::prec_full ? sizeof(cx<T>) : 2 * sizeof(_Float16);
-    scratch.ensure((size_t)std::min(chunk, batch) * n * mid);
+    if (w.p && PREC != p2::prec_full)
+        throw std::logic_error("half-precision transforms keep their own intermediate");
+    const long long chunk = rounds_in(w, this->chunk, (size_t)n * mid, 1);
+    void* midbuf          = w.p;
+    if (!w.p)
+    {
+        scratch.ensure((size_t)std::min(chunk, batch) * n * mid);
+        midbuf = scratch.ptr;
+    }
     if (PREC != p2::prec_full)
         scales.ensure((size_t)(std::min(chunk, batch) << L1 >> gshift) * sizeof(int));
     for (long long b0 = 0; b0 < batch; b0 += chunk)
@@ -1372,11 +1392,11 @@ void pow2_fft<T>::impl::run(bool inverse, const void* in, void* out, long long b
         a.scales = scales.template as<int>();
         a.gshift = gshift;
         a.in     = static_cast<const char*>(in) + b0 * n * es;
-        a.out    = scratch.ptr;
+        a.out    = midbuf;
         a.tw     = t;
         a.lo     = L1;
         p2::launch_any<T, role_col, PREC, p2::min_split_log, ML>(inverse, L2, a, nb << L1, s);
-        a.in  = scratch.ptr;
+        a.in  = midbuf;
         a.out = static_cast<char*>(out) + b0 * n * es;
         a.tw  = t + o_row;
         a.lo  = L2;
@@ -1385,18 +1405,18 @@ void pow2_fft<T>::impl::run(bool inverse, const void* in, void* out, long long b
 }
 
 template <typename T>
-void pow2_fft<T>::run(bool inverse, const void* in, void* out, long long batch, hipStream_t s)
+void pow2_fft<T>::run(bool inverse, const void* in, void* out, long long batch, hipStream_t s, workspace w)
 {
     if (batch <= 0)
         return;
     if constexpr (sizeof(T) == 4)
     {
         if (p->prec == p2::prec_bf16)
-            return p->template run<p2::prec_bf16>(inverse, in, out, batch, s);
+            return p->template run<p2::prec_bf16>(inverse, in, out, batch, s, w);
         if (p->prec == p2::prec_f16)
-            return p->template run<p2::prec_f16>(inverse, in, out, batch, s);
+            return p->template run<p2::prec_f16>(inverse, in, out, batch, s, w);
     }
-    p->template run<p2::prec_full>(inverse, in, out, batch, s);
+    p->template run<p2::prec_full>(inverse, in, out, batch, s, w);
 }
 
 template <typename T>

@@ -24,7 +24,7 @@ public:
     ~pow2_fft();
     // batch transforms: in and out hold batch*n complex values each, item b at offset b*n; in == out
     // is allowed. No scaling (the inverse is unnormalised).
-    void run(bool inverse, const void* in, void* out, long long batch, hipStream_t s);
+    void run(bool inverse, const void* in, void* out, long long batch, hipStream_t s, workspace w = {});
     // lengths whose transform along a strided axis runs in one kernel (2^7 .. the one-kernel maximum)
     static bool supports_axis(size_t n);
     // transforms along a strided axis, in == out allowed: element (o, i, c) at ((o*n + i) << inner_log) + c
