@@ -193,6 +193,17 @@ KFR_AMD_API int kfr_amd_dft_execute_batch_f32(KFR_DFT_PLAN_F32* plan, kfr_c32* o
                                               size_t batch, int inverse, void* stream);
 KFR_AMD_API int kfr_amd_dft_execute_batch_f64(KFR_DFT_PLAN_F64* plan, kfr_c64* out, const kfr_c64* in,
                                               size_t batch, int inverse, void* stream);
+
+/* kfr_amd_dft_execute_batch_f32 that allocates nothing: no counterpart in the reference; for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). The four-step and multi-pass intermediates live in one static 32 MiB device array
+ * per GPU instead of the plan's scratch, in rounds that fit it. Refused with an error (return -1): one
+ * item whose intermediates exceed 32 MiB (f32 four-step lengths above 2^22 points, the multi-pass
+ * lengths from 2^24, mixed-radix products above 4,194,304 points) and Bluestein lengths whose transform
+ * needs more than one workgroup. Calls that overlap on one GPU share the array and race; the plan's own
+ * executions do not use it. */
+KFR_AMD_API int kfr_amd_dft_execute_batch_noalloc_f32(KFR_DFT_PLAN_F32* plan, kfr_c32* out, const kfr_c32* in,
+                                                      size_t batch, int inverse, void* stream);
 KFR_AMD_API int kfr_amd_dft_real_execute_batch_f32(KFR_DFT_REAL_PLAN_F32* plan, void* out, const void* in,
                                                    size_t batch, int inverse, void* stream);
 KFR_AMD_API int kfr_amd_dft_real_execute_batch_f64(KFR_DFT_REAL_PLAN_F64* plan, void* out, const void* in,

@@ -95,6 +95,7 @@ def _proto(lib):
     f("kfr_amd_progressive_delete", None, vp)
     f("kfr_amd_dft_progressive_steps", c_size, C.c_int, c_size, C.c_int)
     f("kfr_amd_dft_real_progressive_steps", c_size, C.c_int, c_size, C.c_int)
+    f("kfr_amd_dft_execute_batch_noalloc_f32", C.c_int, vp, vp, vp, c_size, C.c_int, vp)
     f("kfr_amd_convolve", C.c_long, C.c_int, C.c_int, vp, c_size, vp, c_size, vp)
     f("kfr_amd_conv_create", vp, C.c_int, c_size, c_size)
     f("kfr_amd_conv_fft_size", c_size, vp)

@@ -25,7 +25,8 @@ COMMON = ["-O3", "-std=c++20", "-fPIC", "-fvisibility=hidden", "-Wall", "-Wno-un
           "-Wno-unknown-pragmas"]
 HIPFLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-munsafe-fp-atomics"]
-SOURCES = ["plan_desc.cpp", "capi.cpp", "fft_engine.hip", "fft_pow2.hip", "fft_batch.hip", "fft_md.hip", "fft_mixed.hip"]
+SOURCES = ["plan_desc.cpp", "capi.cpp", "fft_engine.hip", "fft_pow2.hip", "fft_batch.hip", "fft_md.hip", "fft_mixed.hip",
+           "fft_noalloc.hip"]
 HEADERS = ["plan_desc.hpp", "fft_engine.hpp", "fft_core.hpp", "fft_device.hpp", "fft_pow2.hpp", "fft_batch.hpp", "fft_md.hpp", "fft_mixed.hpp", "fft_timing.hpp"]
 
 

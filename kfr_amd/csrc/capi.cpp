@@ -1197,6 +1197,21 @@ extern "C"
             },
             -1);
     }
+    int kfr_amd_dft_execute_batch_noalloc_f32(KFR_DFT_PLAN_F32* plan, kfr_c32* out, const kfr_c32* in, size_t batch,
+                                              int inverse, void* stream)
+    {
+        return try_fn(
+            [&]
+            {
+                auto* o = dft_cast<float>(plan);
+                if (!o->g)
+                    throw std::invalid_argument("the allocation-free execute takes one-dimensional plans");
+                o->g->execute_workspace(out, in, (long long)batch, inverse != 0, as_stream(stream),
+                                        noalloc_workspace());
+                return 0;
+            },
+            -1);
+    }
     size_t kfr_amd_dft_real_get_complex_size_f32(KFR_DFT_REAL_PLAN_F32* plan)
     {
         return try_fn([&] { return real_cast<float>(plan)->complex_size(); }, (size_t)0);

@@ -56,6 +56,12 @@ inline void* workspace_part(const workspace& w, int b, int buffers)
     return static_cast<char*>(w.p) + (size_t)b * each;
 }
 
+// The workspace of the allocation-free entry point: a static device array of noalloc_scratch_bytes
+// (fft_noalloc.hip), its address on the current device looked up on first use there. Calls that use
+// it and overlap on one GPU race.
+constexpr size_t noalloc_scratch_bytes = (size_t)32 << 20;
+workspace noalloc_workspace();
+
 // Owned device allocation
 struct dev_mem
 {
