@@ -3,6 +3,7 @@
 //   emu_main real f32|f64 n fmt batch inverse in.bin out.bin
 //   emu_main dct  f32|f64 n batch inverse in.bin out.bin
 //   emu_main prog f32|f64 n want inverse in.bin out.bin   (progressive units of one transform)
+//   emu_main ws   f32|f64 n batch inverse mb in.bin out.bin (execute with an mb-MiB caller workspace)
 // The input is copied into a buffer of exactly its size and the output buffer holds exactly the
 // output, so under AddressSanitizer every access beyond either is reported.
 #include "../../kfr_amd/csrc/fft_engine.hpp"
@@ -34,6 +35,37 @@ static int run(const std::string& kind, int argc, char** argv)
 {
     const size_t n = std::strtoull(argv[3], nullptr, 10);
     int a          = 4;
+    if (kind == "ws")
+    {
+        const long long batch = std::atoll(argv[4]);
+        const bool inverse    = std::atoi(argv[5]) != 0;
+        const size_t wbytes   = (size_t)std::atoll(argv[6]) << 20;
+        const std::vector<char> in = read_file(argv[7]);
+        const size_t bytes         = (size_t)batch * n * 2 * sizeof(T);
+        if (in.size() != bytes)
+            return 2;
+        char* din  = static_cast<char*>(std::malloc(bytes));
+        char* dout = static_cast<char*>(std::malloc(bytes));
+        void* ws   = std::malloc(wbytes);
+        std::memcpy(din, in.data(), bytes);
+        try
+        {
+            gpu_dft<T>(n).execute_workspace(dout, din, batch, inverse, nullptr, workspace{ ws, wbytes });
+        }
+        catch (const std::exception& e)
+        {
+            std::fprintf(stderr, "error: %s\n", e.what());
+            return 3;
+        }
+        FILE* f = std::fopen(argv[8], "wb");
+        if (!f || std::fwrite(dout, 1, bytes, f) != bytes)
+            return 4;
+        std::fclose(f);
+        std::free(din);
+        std::free(dout);
+        std::free(ws);
+        return 0;
+    }
     if (kind == "prog")
     {
         const int want     = std::atoi(argv[4]);
@@ -127,7 +159,7 @@ int main(int argc, char** argv)
         return 2;
     }
     const std::string kind = argv[1], t = argv[2];
-    if (kind != "dft" && kind != "real" && kind != "dct" && kind != "prog")
+    if (kind != "dft" && kind != "real" && kind != "dct" && kind != "prog" && kind != "ws")
         return 2;
     return t == "f64" ? run<double>(kind, argc, argv) : run<float>(kind, argc, argv);
 }
