@@ -205,6 +205,16 @@ def precision_config(torch, kfr_amd, stream, reps: int):
     for name in ("bf16", "f16"):
         out[name]["speedup_vs_f32"] = round(out["f32"]["ms"] / out[name]["ms"], 3)
     res.update(out)
+    # MFMA utilisation of the half-precision kernels of this workload, from the committed counter run
+    # (rocprofv3 cannot run inside this process); per dispatch, in percent of the chip's SIMD cycles
+    from kfr_amd.rocprof_summary import read_table
+    mfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "half_c2c_2p20x64_mfma_counters.md")
+    rows = [r for r in read_table(mfile) if r["kernel"].startswith("k_pow2<float") and r["kernel"].endswith((", 1>", ", 2>"))]
+    if rows:
+        res["mfma_counters"] = {"source": "profiles/half_c2c_2p20x64_mfma_counters.md",
+                                "kernels": {r["kernel"]: {"mfma_util_pct": float(r["MfmaUtil"]),
+                                                          "valu_busy_pct": float(r["VALUBusy"]),
+                                                          "wait_share": float(r["wait share"])} for r in rows}}
     return res
 
 
@@ -331,7 +341,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true", help="sizes up to 2^20 only, C2C only")
     ap.add_argument("--only", default=None,
-                    help="comma-separated subset of configurations kind:dtype:n (e.g. c2c:f32:4096), for profiling")
+                    help="comma-separated subset of configurations kind:dtype:n (e.g. c2c:f32:4096), for profiling; "
+                         "the words uneven, precision and extra add those measurements to the subset")
     ap.add_argument("--no-uneven", action="store_true", help="skip the mixed-length batch measurement")
     ap.add_argument("--no-precision", action="store_true", help="skip the half-precision measurement")
     ap.add_argument("--no-extra", action="store_true",
@@ -353,8 +364,11 @@ def main() -> int:
     torch.cuda.set_device(dev)
     stream = torch.cuda.current_stream()
     cfgs = sweep_configs(args.quick)
+    extras = set()
     if args.only:
-        keep = {tuple(x.split(":")) for x in args.only.split(",")}
+        words = args.only.split(",")
+        extras = {w for w in words if w in ("uneven", "precision", "extra")}
+        keep = {tuple(x.split(":")) for x in words if x not in extras}
         cfgs = [c for c in cfgs if (c[0], c[1], str(c[2])) in keep]
         if not cfgs:
             print(json.dumps({"error": f"no configuration matches --only {args.only}"}))
@@ -529,15 +543,15 @@ def main() -> int:
     lds_bytes = sum(alg_bytes("c2c", "f32", p["n"], p["batch"]) for p in lds)
 
     uneven = None
-    if not args.no_uneven and not args.only:
+    if not args.no_uneven and (not args.only or "uneven" in extras):
         uneven = uneven_batch(torch, kfr_amd, stream, max(3, args.steps), check=True)
 
     precision = None
-    if not args.no_precision and not args.only:
+    if not args.no_precision and (not args.only or "precision" in extras):
         precision = precision_config(torch, kfr_amd, stream, max(3, args.steps))
 
     round_trip = conv = md = None
-    if not args.no_extra and not args.only:
+    if not args.no_extra and (not args.only or "extra" in extras):
         round_trip = round_trip_config(torch, kfr_amd, stream, max(3, args.steps))
         conv = conv_config(torch, kfr_amd, stream, max(3, args.steps))
         md = md_config(torch, kfr_amd, stream, max(3, args.steps))

@@ -6,8 +6,9 @@
 // conj(DFT(conj x)), so the kernel carries one set of butterflies. The queue reset and the kernel
 // are captured once into a hipGraph and replayed.
 //
-// The transforms-per-item of each length comes from a small autotuner that times the candidates on
-// the device (or, without tuning, from the occupancy limits).
+// An autotuner times candidate schedules of each length on the device: the order of its radix passes,
+// the LDS padding of its tile, and the transforms per item (without tuning: the planner's order, one
+// padding slot per 16 values, and as many transforms as the occupancy limits allow).
 #include "fft_batch.hpp"
 
 #include "fft_core.hpp"
@@ -15,6 +16,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <stdexcept>
@@ -137,11 +139,54 @@ struct host_len
     size_t n  = 0;
     size_t m  = 0; // LDS transform length (n, or the Bluestein M)
     bool bs   = false;
-    stockham S;
-    int tpb_max = 1;
-    int tpb     = 1;
+    stockham S;                            // schedule of the chosen pass order
+    std::vector<std::vector<int>> orders;  // candidate pass orders (the planner's first)
+    int order    = 0;                      // chosen pass order
+    int padshift = 4;                      // chosen LDS padding: one slot per 2^padshift values
+    int tpb_max  = 1;
+    int tpb      = 1;
     long long tw = 0, chirp = 0, H = 0;
 };
+
+// LDS paddings tried: one slot per 16 or per 32 values, or none (the tile is sized for one per 16)
+constexpr int padshifts[] = { 4, 5, 30 };
+
+// Pass orders tried for one length: the planner's (powers of two, largest first, then 7, 5, 3 and the
+// direct primes), its reverse, and the odd factors ahead of the powers of two. Every order of one
+// radix multiset has the same threads per transform and table size; each pass reads its own stride
+// and twiddles from the length's descriptor, so any order computes the same transform.
+inline std::vector<std::vector<int>> radix_orders(const std::vector<int>& r)
+{
+    std::vector<std::vector<int>> out{ r };
+    std::vector<int> rev(r.rbegin(), r.rend());
+    std::vector<int> odd_first;
+    for (int x : r)
+        if (x % 2 != 0)
+            odd_first.push_back(x);
+    for (int x : r)
+        if (x % 2 == 0)
+            odd_first.push_back(x);
+    for (auto& o : { rev, odd_first })
+        if (std::find(out.begin(), out.end(), o) == out.end())
+            out.push_back(o);
+    return out;
+}
+
+template <typename T>
+void set_order(host_len<T>& h, int k)
+{
+    h.order = k;
+    if (!stockham_from_radices((long long)h.m, h.orders[k], h.S, pmax_of<T>, true))
+        throw std::logic_error("batch: pass order does not factor its length");
+}
+
+// forced choices for tests and measurements: KFR_AMD_BATCH_ORDER (index of the pass order, modulo the
+// number of candidates) and KFR_AMD_BATCH_PADSHIFT (4, 5 or 30); -1 when unset
+inline int env_choice(const char* name)
+{
+    const char* e = std::getenv(name);
+    return e ? std::atoi(e) : -1;
+}
 
 inline size_t align2(size_t v) { return (v + 1) / 2 * 2; }
 
@@ -198,7 +243,8 @@ len_desc make_desc(const host_len<T>& h, int tpb)
 {
     len_desc d;
     check_pass_radices<T>(h.S, true);
-    d.G         = make_geom(h.S, tpb, 0, 1, (int)h.m);
+    d.G          = make_geom(h.S, tpb, 0, 1, (int)h.m);
+    d.G.padshift = h.padshift;
     d.tw        = h.tw;
     d.chirp     = h.chirp;
     d.H         = h.H;
@@ -207,10 +253,15 @@ len_desc make_desc(const host_len<T>& h, int tpb)
     return d;
 }
 
-// process-wide cache of tuned transforms-per-item, keyed by (element size, length)
-inline std::map<std::pair<int, size_t>, int>& tuned()
+struct tuning
 {
-    static std::map<std::pair<int, size_t>, int> m;
+    int order = 0, padshift = 4, tpb = 1;
+};
+
+// process-wide cache of tuned schedules, keyed by (element size, length)
+inline std::map<std::pair<int, size_t>, tuning>& tuned()
+{
+    static std::map<std::pair<int, size_t>, tuning> m;
     return m;
 }
 inline std::mutex& tuned_mu()
@@ -219,81 +270,133 @@ inline std::mutex& tuned_mu()
     return mu;
 }
 
-// time each candidate transforms-per-item for one length on synthetic data; returns the fastest per
-// transform
+// Times candidate schedules of one length on synthetic data and leaves the fastest per transform in h:
+// first every pass order with every LDS padding at the largest transforms-per-item, then the
+// transforms per item (the largest and up to four halvings of it, and 1) with the winning order and
+// padding. Each candidate runs one untimed launch and three timed ones (median). Forced choices
+// (KFR_AMD_BATCH_ORDER, KFR_AMD_BATCH_PADSHIFT) are kept and not searched.
 template <typename T>
-int autotune(const host_len<T>& h, const dev_mem& tab, hipStream_t s)
+void autotune(host_len<T>& h, int force_order, int force_ps, hipStream_t s)
 {
-    std::vector<int> cand;
+    std::vector<int> orders, pads, tpbs;
+    for (int k = 0; k < (int)h.orders.size(); ++k)
+        if (force_order < 0 || k == force_order % (int)h.orders.size())
+            orders.push_back(k);
+    for (int ps : padshifts)
+        if (force_ps < 0 || ps == force_ps)
+            pads.push_back(ps);
+    if (pads.empty())
+        pads.push_back(4);
     for (int k = 0; k < max_candidates - 1; ++k)
         if ((h.tpb_max >> k) >= 1)
-            cand.push_back(h.tpb_max >> k);
-    cand.push_back(1);
-    std::sort(cand.begin(), cand.end());
-    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
-    if (cand.size() == 1)
-        return cand[0];
-    hipEvent_t e0, e1;
-    hip_check(hipEventCreate(&e0), "event");
-    hip_check(hipEventCreate(&e1), "event");
+            tpbs.push_back(h.tpb_max >> k);
+    tpbs.push_back(1);
+    std::sort(tpbs.begin(), tpbs.end());
+    tpbs.erase(std::unique(tpbs.begin(), tpbs.end()), tpbs.end());
+
+    // one table for the candidates: Bluestein chirp and spectrum (order-independent), then the
+    // twiddles of the order being timed
+    const size_t c0 = 0, h0 = h.bs ? align2(h.n) : 0, t0 = h.bs ? h0 + align2(h.m) : 0;
+    dev_mem tab((t0 + align2((size_t)std::max<long long>(1, h.S.tw_count))) * sizeof(cx<T>));
+    if (h.bs)
+        bluestein_tables<T>(h.n, h.m, tab.as<cx<T>>() + c0, tab.as<cx<T>>() + h0, s);
+    hipEvent_t ev[4];
+    for (auto& e : ev)
+        hip_check(hipEventCreate(&e), "event");
     dev_mem lens(sizeof(len_desc)), counter(sizeof(unsigned)), items, members, offs, data;
-    int best     = cand.back();
-    double bestt = 1e300;
+    int loaded_order = -1;
+    auto time_one = [&](int order, int ps, int tpb) -> double
+    {
+        if (order != loaded_order)
+        {
+            set_order(h, order);
+            gen_stockham_twiddles(h.S, tab.as<cx<T>>() + t0, s);
+            loaded_order = order;
+        }
+        h.padshift          = ps;
+        const size_t lb     = lds_of<T>(h.m, tpb);
+        const unsigned grid = grid_for<T>(lb);
+        const int nitems    = (int)(2 * grid);
+        const long long nt  = (long long)nitems * tpb;
+        std::vector<work_item> it(nitems);
+        std::vector<int> mem(nt);
+        std::vector<long long> of(nt);
+        for (int i = 0; i < nitems; ++i)
+            it[i] = { 0, 0, i * tpb, tpb };
+        for (long long j = 0; j < nt; ++j)
+        {
+            mem[j] = (int)j;
+            of[j]  = j * (long long)h.n;
+        }
+        host_len<T> hh = h;
+        hh.tw          = (long long)t0;
+        hh.chirp       = (long long)c0;
+        hh.H           = (long long)h0;
+        const len_desc d = make_desc(hh, tpb);
+        items.ensure(it.size() * sizeof(work_item));
+        members.ensure(mem.size() * sizeof(int));
+        offs.ensure(of.size() * sizeof(long long));
+        data.ensure((size_t)nt * h.n * sizeof(cx<T>));
+        hip_check(hipMemcpyAsync(lens.ptr, &d, sizeof(d), hipMemcpyHostToDevice, s), "tuning upload");
+        hip_check(hipMemcpyAsync(items.ptr, it.data(), it.size() * sizeof(work_item), hipMemcpyHostToDevice, s),
+                  "tuning upload");
+        hip_check(hipMemcpyAsync(members.ptr, mem.data(), mem.size() * sizeof(int), hipMemcpyHostToDevice, s),
+                  "tuning upload");
+        hip_check(hipMemcpyAsync(offs.ptr, of.data(), of.size() * sizeof(long long), hipMemcpyHostToDevice, s),
+                  "tuning upload");
+        hip_check(hipMemsetAsync(data.ptr, 0, (size_t)nt * h.n * sizeof(cx<T>), s), "tuning data");
+        launch<T>(lens, items, nitems, members, offs, tab, data.ptr, data.ptr, counter, lb, grid, s); // warm
+        hip_check(hipEventRecord(ev[0], s), "event");
+        for (int r = 0; r < 3; ++r)
+        {
+            launch<T>(lens, items, nitems, members, offs, tab, data.ptr, data.ptr, counter, lb, grid, s);
+            hip_check(hipEventRecord(ev[r + 1], s), "event");
+        }
+        hip_check(hipEventSynchronize(ev[3]), "tuning");
+        float ms[3];
+        for (int r = 0; r < 3; ++r)
+            hip_check(hipEventElapsedTime(&ms[r], ev[r], ev[r + 1]), "event");
+        std::sort(ms, ms + 3);
+        return (double)ms[1] / (double)nt;
+    };
     try
     {
-        for (int tpb : cand)
-        {
-            const size_t lb     = lds_of<T>(h.m, tpb);
-            const unsigned grid = grid_for<T>(lb);
-            const int nitems    = (int)(2 * grid);
-            const long long nt  = (long long)nitems * tpb;
-            std::vector<work_item> it(nitems);
-            std::vector<int> mem(nt);
-            std::vector<long long> of(nt);
-            for (int i = 0; i < nitems; ++i)
-                it[i] = { 0, 0, i * tpb, tpb };
-            for (long long j = 0; j < nt; ++j)
+        double best = 1e300;
+        tuning b{ orders[0], pads[0], h.tpb_max };
+        for (int o : orders)
+            for (int ps : pads)
             {
-                mem[j] = (int)j;
-                of[j]  = j * (long long)h.n;
+                const double t = time_one(o, ps, h.tpb_max);
+                if (t < best)
+                {
+                    best = t;
+                    b    = { o, ps, h.tpb_max };
+                }
             }
-            const len_desc d = make_desc(h, tpb);
-            items.ensure(it.size() * sizeof(work_item));
-            members.ensure(mem.size() * sizeof(int));
-            offs.ensure(of.size() * sizeof(long long));
-            data.ensure((size_t)nt * h.n * sizeof(cx<T>));
-            hip_check(hipMemcpyAsync(lens.ptr, &d, sizeof(d), hipMemcpyHostToDevice, s), "tuning upload");
-            hip_check(hipMemcpyAsync(items.ptr, it.data(), it.size() * sizeof(work_item), hipMemcpyHostToDevice, s),
-                      "tuning upload");
-            hip_check(hipMemcpyAsync(members.ptr, mem.data(), mem.size() * sizeof(int), hipMemcpyHostToDevice, s),
-                      "tuning upload");
-            hip_check(hipMemcpyAsync(offs.ptr, of.data(), of.size() * sizeof(long long), hipMemcpyHostToDevice, s),
-                      "tuning upload");
-            hip_check(hipMemsetAsync(data.ptr, 0, (size_t)nt * h.n * sizeof(cx<T>), s), "tuning data");
-            launch<T>(lens, items, nitems, members, offs, tab, data.ptr, data.ptr, counter, lb, grid, s); // warm
-            hip_check(hipEventRecord(e0, s), "event");
-            launch<T>(lens, items, nitems, members, offs, tab, data.ptr, data.ptr, counter, lb, grid, s);
-            hip_check(hipEventRecord(e1, s), "event");
-            hip_check(hipEventSynchronize(e1), "tuning");
-            float ms = 0;
-            hip_check(hipEventElapsedTime(&ms, e0, e1), "event");
-            const double per = (double)ms / (double)nt;
-            if (per < bestt)
+        for (int tpb : tpbs)
+        {
+            if (tpb == h.tpb_max)
+                continue;
+            const double t = time_one(b.order, b.padshift, tpb);
+            if (t < best)
             {
-                bestt = per;
-                best  = tpb;
+                best  = t;
+                b.tpb = tpb;
             }
         }
+        hip_check(hipStreamSynchronize(s), "tuning");
+        set_order(h, b.order);
+        h.padshift = b.padshift;
+        h.tpb      = b.tpb;
     }
     catch (...)
     {
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        for (auto& e : ev)
+            (void)hipEventDestroy(e);
         throw;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    return best;
+    for (auto& e : ev)
+        (void)hipEventDestroy(e);
 }
 
 } // namespace bt
@@ -356,6 +459,7 @@ gpu_batch<T>::gpu_batch(size_t count, const size_t* sizes, const int* inverse, b
         if (!make_stockham((long long)h.m, h.S, pmax_of<T>, true) || h.S.tpt > block || lds_of<T>(h.m, 1) > lds_cap)
             throw std::invalid_argument("transform length " + std::to_string(n) +
                                         " does not fit one workgroup of the batch kernel");
+        h.orders = radix_orders(h.S.radix);
         h.tpb_max = std::max(1, block / h.S.tpt);
         while (h.tpb_max > 1 && lds_of<T>(h.m, h.tpb_max) > lds_cap)
             --h.tpb_max;
@@ -363,6 +467,43 @@ gpu_batch<T>::gpu_batch(size_t count, const size_t* sizes, const int* inverse, b
         L.push_back(h);
     }
     p->total = (size_t)pos;
+    // schedules: tuned (cached per process), forced, or the planner's order, padding of one slot per 16
+    // values and the largest transforms-per-item that fits
+    const int force_order = env_choice("KFR_AMD_BATCH_ORDER");
+    const int force_ps    = env_choice("KFR_AMD_BATCH_PADSHIFT");
+    if (force_ps >= 0 && std::find(std::begin(padshifts), std::end(padshifts), force_ps) == std::end(padshifts))
+        throw std::invalid_argument("KFR_AMD_BATCH_PADSHIFT must be 4, 5 or 30");
+    int tuned_count = 0;
+    for (auto& h : L)
+    {
+        h.tpb = h.tpb_max;
+        if (force_order >= 0)
+            set_order(h, force_order % (int)h.orders.size());
+        if (force_ps >= 0)
+            h.padshift = force_ps;
+        if (!autotune)
+            continue;
+        const std::pair<int, size_t> key{ (int)sizeof(T), h.n };
+        if (force_order < 0 && force_ps < 0)
+        {
+            std::lock_guard<std::mutex> lock(tuned_mu());
+            auto f = tuned().find(key);
+            if (f != tuned().end())
+            {
+                set_order(h, f->second.order);
+                h.padshift = f->second.padshift;
+                h.tpb      = std::min(f->second.tpb, h.tpb_max);
+                continue;
+            }
+        }
+        bt::autotune<T>(h, force_order, force_ps, s);
+        ++tuned_count;
+        if (force_order < 0 && force_ps < 0)
+        {
+            std::lock_guard<std::mutex> lock(tuned_mu());
+            tuned()[key] = { h.order, h.padshift, h.tpb };
+        }
+    }
     // tables: per length its twiddles, and for Bluestein its chirp and spectrum
     size_t tabsz = 0;
     for (auto& h : L)
@@ -386,28 +527,6 @@ gpu_batch<T>::gpu_batch(size_t count, const size_t* sizes, const int* inverse, b
             bluestein_tables<T>(h.n, h.m, tb + h.chirp, tb + h.H, s);
     }
     hip_check(hipStreamSynchronize(s), "batch tables");
-    // transforms per item: tuned (cached per process) or the largest that fits
-    int tuned_count = 0;
-    for (auto& h : L)
-    {
-        h.tpb = h.tpb_max;
-        if (!autotune)
-            continue;
-        const std::pair<int, size_t> key{ (int)sizeof(T), h.n };
-        {
-            std::lock_guard<std::mutex> lock(tuned_mu());
-            auto f = tuned().find(key);
-            if (f != tuned().end())
-            {
-                h.tpb = std::min(f->second, h.tpb_max);
-                continue;
-            }
-        }
-        h.tpb = bt::autotune<T>(h, p->tab, s);
-        ++tuned_count;
-        std::lock_guard<std::mutex> lock(tuned_mu());
-        tuned()[key] = h.tpb;
-    }
     // group transforms by (length, direction) and cut the groups into items, costliest first
     std::map<std::pair<int, int>, std::vector<int>> groups;
     for (size_t i = 0; i < count; ++i)
@@ -458,6 +577,17 @@ gpu_batch<T>::gpu_batch(size_t count, const size_t* sizes, const int* inverse, b
     p->info = "gpu(gfx950): batch transforms=" + std::to_string(count) + " lengths=" + std::to_string(L.size()) +
               " items=" + std::to_string(p->nitems) + " workgroups=" + std::to_string(p->grid) +
               " lds=" + std::to_string(p->lds_bytes) + (autotune ? " autotuned=" + std::to_string(tuned_count) : "");
+    // schedule choices over the distinct lengths: pass order (planner's, reversed, odd factors first),
+    // LDS padding (1/16, 1/32, none)
+    int by_order[3] = { 0, 0, 0 }, by_pad[3] = { 0, 0, 0 };
+    for (auto& h : L)
+    {
+        ++by_order[std::min(h.order, 2)];
+        ++by_pad[h.padshift == 4 ? 0 : h.padshift == 5 ? 1 : 2];
+    }
+    p->info += " orders=" + std::to_string(by_order[0]) + "/" + std::to_string(by_order[1]) + "/" +
+               std::to_string(by_order[2]) + " padding=" + std::to_string(by_pad[0]) + "/" +
+               std::to_string(by_pad[1]) + "/" + std::to_string(by_pad[2]);
 }
 
 template <typename T>

@@ -467,7 +467,8 @@ inline lds_geom make_geom(const stockham& S, int tpb, int mapping, int ls, int l
     static const int padshift = []
     {
         const char* e = std::getenv("KFR_AMD_LDS_PADSHIFT");
-        return e ? std::max(1, std::min(30, atoi(e))) : 4;
+        // the tiles are sized for one padding slot per 16 values (lds_padded_elems): sparser only
+        return e ? std::max(4, std::min(30, atoi(e))) : 4;
     }();
     G.padshift = padshift;
     for (int p = 0; p < G.np; ++p)

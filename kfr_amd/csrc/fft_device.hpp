@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
+#include <utility>
 
 namespace kfr_amd
 {
@@ -213,6 +215,14 @@ __device__ __forceinline__ void bfly16(cx<T>* v)
         v[i] = x[i];
 }
 
+// f(std::integral_constant<int, I>) for I = 0 .. N-1, unrolled at compile time
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+    [&]<int... I>(std::integer_sequence<int, I...>) { (f(std::integral_constant<int, I>{}), ...); }(
+        std::make_integer_sequence<int, N>{});
+}
+
 // Odd prime R: conjugate-pair form. X[0] = sum; for k>0 X[k] = x0 + sum_{n=1}^{h}
 // (x_n + x_{R-n}) cos(2 pi n k/R) -+ i (x_n - x_{R-n}) sin(2 pi n k/R).
 template <int R, bool INV, typename T>
@@ -229,20 +239,21 @@ __device__ __forceinline__ void bfly_odd(cx<T>* v)
         sum      = cadd(sum, s[n - 1]);
     }
     v[0] = sum;
-#pragma unroll
-    for (int k = 1; k <= h; ++k)
+    // the cosines and sines are template constants: evaluated by the compiler, never at run time
+    static_for<h>([&](auto km1)
     {
+        constexpr int k = decltype(km1)::value + 1;
         cx<T> a = x0, b = { T(0), T(0) };
-#pragma unroll
-        for (int n = 1; n <= h; ++n)
+        static_for<h>([&](auto nm1)
         {
-            const T c = T(ccos2pi(long(n) * k, R));
-            const T sn = T(csin2pi(long(n) * k, R));
+            constexpr int n  = decltype(nm1)::value + 1;
+            constexpr T c    = T(ccos2pi(long(n) * k, R));
+            constexpr T sn   = T(csin2pi(long(n) * k, R));
             a.re += s[n - 1].re * c;
             a.im += s[n - 1].im * c;
             b.re += d[n - 1].re * sn;
             b.im += d[n - 1].im * sn;
-        }
+        });
         // forward: X[k] = a - i b, X[R-k] = a + i b ; inverse: swapped signs
         cx<T> ib = { -b.im, b.re };
         if constexpr (INV)
@@ -255,7 +266,7 @@ __device__ __forceinline__ void bfly_odd(cx<T>* v)
             v[k]     = csub(a, ib);
             v[R - k] = cadd(a, ib);
         }
-    }
+    });
 }
 
 template <int R, bool INV, typename T>

@@ -119,7 +119,8 @@ struct geom
         // half of each row in LDS at a time: twice the rows per workgroup and twice the run length of
         // the transposed store (measured 2-6% faster for rows of 2^11..2^13 points)
         constexpr int grow = ROLE == role_row && PREC == 0 && S1::LE >= log_e<T> ? 2 : 1;
-        int w = 16;
+        // half-precision passes hold 32 butterflies: 128-byte runs of 4-byte values
+        int w = ROLE == role_pass && PREC != 0 ? 32 : 16;
         while (w > 1 && (w * S1::N > grow * lds_budget<T> || w * S1::TPT > grow * 1024))
             w /= 2;
         // runs longer than a 128-byte line buy nothing, two workgroups per CU do (measured for f64 at
@@ -211,6 +212,12 @@ struct args
     int gshift       = 0; // half-precision row kernel: log2 of the column kernel's columns per workgroup
     int ns_log       = 0; // multi-pass: log2 of the stride Ns of this pass (twiddles W_{Ns*R} from ta, tb)
     unsigned nblk    = 0; // workgroups with work (the grid of split kernels is a multiple of 8)
+    // half-precision multi-pass: the input is the previous pass's binary16 intermediate, element o of
+    // item b scaled by in_scales[workgroup of the previous pass that wrote it]; that pass had radix
+    // 2^in_l, stride 2^in_ns and 2^in_wlog butterflies per workgroup
+    const int* in_scales = nullptr;
+    int in_l = 0, in_ns = 0, in_wlog = 0;
+    int out_mid = 0; // write a binary16 intermediate with one power-of-two scale per workgroup (a.scales)
 };
 
 // ---- storage precision of the data in HBM (the arithmetic is T throughout)
@@ -348,29 +355,36 @@ __device__ __forceinline__ f32x16 mfma16_group(const cx<float> (&g)[8], const mf
     return acc;
 }
 
-// value of lane l ^ 32: v_permlane32_swap exchanges the upper half-wave of its first operand with the
-// lower half-wave of its second, without going through LDS
-__device__ __forceinline__ float xchg32(float x)
+// v_permlane32_swap on the pair (a, b): the upper half-wave of a and the lower half-wave of b trade
+// places. Lane l < 32 keeps its a and receives the a of lane l + 32 in b; lane l >= 32 keeps its b
+// and receives the b of lane l - 32 in a. One instruction moves both registers.
+__device__ __forceinline__ void swap32(float& a, float& b)
 {
-    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
-    return __uint_as_float(__lane_id() < 32 ? r[1] : r[0]);
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    a            = __uint_as_float(r[0]);
+    b            = __uint_as_float(r[1]);
 }
 
 // Wave-cooperative radix-16 butterfly: every lane of the wave holds the 16 inputs of its own
-// butterfly in u and gets its 16 outputs back. Lanes l and l+32 exchange halves so that each holds
-// inputs [8h, 8h+8) of butterflies l%32 (group 0) and 32 + l%32 (group 1), both groups run on MFMA,
-// and a second exchange returns every output to the lane that owns the butterfly.
+// butterfly in u and gets its 16 outputs back. Swapping the pair (u[s], u[8+s]) leaves lane l < 32
+// with input s of butterflies l and l + 32, and lane l >= 32 with input 8 + s of butterflies l - 32 and
+// l: inputs [8h, 8h+8) of butterflies l%32 (group 0) and 32 + l%32 (group 1), the B operand layout.
+// Both groups run on MFMA; lane h then holds outputs m(h, q) of both groups, and swapping each pair
+// (group 0, group 1) returns outputs m(0, q) and m(1, q) of its own butterfly to every lane.
 __device__ __forceinline__ void bfly16_mfma(cx<float> (&u)[16], const mfma16& K)
 {
-    const int h = __lane_id() >> 5;
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+    {
+        swap32(u[s].re, u[8 + s].re);
+        swap32(u[s].im, u[8 + s].im);
+    }
     cx<float> g0[8], g1[8];
 #pragma unroll
     for (int s = 0; s < 8; ++s)
     {
-        const cx<float> snd = h ? u[s] : u[8 + s];
-        const cx<float> rcv = { xchg32(snd.re), xchg32(snd.im) };
-        g0[s]               = h ? rcv : u[s];
-        g1[s]               = h ? u[8 + s] : rcv;
+        g0[s] = u[s];
+        g1[s] = u[8 + s];
     }
     const f32x16 d0 = mfma16_group(g0, K);
     const f32x16 d1 = mfma16_group(g1, K);
@@ -378,15 +392,12 @@ __device__ __forceinline__ void bfly16_mfma(cx<float> (&u)[16], const mfma16& K)
 #pragma unroll
     for (int q = 0; q < 8; ++q)
     {
-        const cx<float> o0  = { d0[q], d0[8 + q] };
-        const cx<float> o1  = { d1[q], d1[8 + q] };
-        const cx<float> snd = h ? o0 : o1;
-        const cx<float> rcv = { xchg32(snd.re), xchg32(snd.im) };
-        const cx<float> own = h ? o1 : o0;
-        const int m0        = q < 4 ? q : 4 + q;     // output index held by h = 0
-        const int m1        = q < 4 ? 4 + q : 8 + q; // output index held by h = 1
-        u[m0]               = h ? rcv : own;
-        u[m1]               = h ? own : rcv;
+        cx<float> a = { d0[q], d0[8 + q] };
+        cx<float> b = { d1[q], d1[8 + q] };
+        swap32(a.re, b.re);
+        swap32(a.im, b.im);
+        u[q < 4 ? q : 4 + q]     = a; // output m(0, q)
+        u[q < 4 ? 4 + q : 8 + q] = b; // output m(1, q)
     }
 }
 
@@ -511,6 +522,31 @@ __device__ __forceinline__ int mid_scale_exp(float mx)
     return mx > 0.f ? 15 - e : 0;
 }
 
+// the scale exponent of all values the workgroup holds (every thread calls it): wave maxima, then one
+// value per wave through LDS; the LDS is free again when it returns
+template <typename T, int E, int BS>
+__device__ __forceinline__ int workgroup_scale(const cx<T> (&v)[E], cx<T>* lds)
+{
+    float mx = 0.f;
+#pragma unroll
+    for (int k = 0; k < E; ++k)
+        mx = fmaxf(mx, fmaxf(fabsf((float)v[k].re), fabsf((float)v[k].im)));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        mx = fmaxf(mx, __shfl_xor(mx, off));
+    float* red = reinterpret_cast<float*>(lds);
+    lds_barrier();
+    if (__lane_id() == 0)
+        red[threadIdx.x >> 6] = mx;
+    lds_barrier();
+    float bm = 0.f;
+#pragma unroll
+    for (int w = 0; w < (BS + 63) / 64; ++w)
+        bm = fmaxf(bm, red[w]);
+    lds_barrier();
+    return mid_scale_exp(bm);
+}
+
 template <typename T, int L, bool INV, int ROLE, int PREC>
 __global__ void __launch_bounds__((geom<T, L, ROLE, PREC>::BS)) k_pow2(args<T> a)
 {
@@ -571,9 +607,35 @@ __global__ void __launch_bounds__((geom<T, L, ROLE, PREC>::BS)) k_pow2(args<T> a
         const long long bl   = g >> a.lo;
         const long long col  = g & ((1ll << a.lo) - 1);
         const long long base = (bl << (a.lo + L)) + col;
+        bool scaled_in       = false;
+        if constexpr (ROLE == role_pass && PREC != prec_full)
+            scaled_in = a.in_scales != nullptr;
+        if (scaled_in)
+        {
+            // binary16 intermediate of the previous pass: element o was written by its butterfly
+            // j' = (o >> (ns' + l')) << ns' | (o mod 2^ns'), i.e. by workgroup (bl 2^(Ltot - l') + j') >> wlog'
+            const long long nsm = (1ll << a.in_ns) - 1;
 #pragma unroll
-        for (int k = 0; k < E; ++k)
-            v[k] = valid ? ld_elem<T, PREC>(a.in, base + ((long long)(t + k * TP) << a.lo)) : cx<T>{ T(0), T(0) };
+            for (int k = 0; k < E; ++k)
+            {
+                v[k] = { T(0), T(0) };
+                if (valid)
+                {
+                    const long long o  = col + ((long long)(t + k * TP) << a.lo);
+                    const long long jp = ((o >> (a.in_ns + a.in_l)) << a.in_ns) | (o & nsm);
+                    const long long gp = (bl << (a.lo + L - a.in_l)) + jp;
+                    const int sc       = a.in_scales[gp >> a.in_wlog];
+                    const cx<T> h      = ld_elem<T, prec_f16>(a.in, base + ((long long)(t + k * TP) << a.lo));
+                    v[k]               = { ldexpf(h.re, -sc), ldexpf(h.im, -sc) };
+                }
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int k = 0; k < E; ++k)
+                v[k] = valid ? ld_elem<T, PREC>(a.in, base + ((long long)(t + k * TP) << a.lo)) : cx<T>{ T(0), T(0) };
+        }
         if constexpr (ROLE == role_pass)
         {
             if (a.ns_log > 0)
@@ -665,6 +727,31 @@ __global__ void __launch_bounds__((geom<T, L, ROLE, PREC>::BS)) k_pow2(args<T> a
     {
         const long long bl  = g >> a.lo;
         const long long col = g & ((1ll << a.lo) - 1);
+        // half precision: an intermediate is written as binary16 scaled by 2^sc, one sc per workgroup
+        int sc = 0;
+        bool mid = false;
+        if constexpr (PREC != prec_full)
+        {
+            mid = a.out_mid != 0;
+            if (mid)
+            {
+                sc = workgroup_scale<T, E, BS>(v, lds);
+                if (tid == 0)
+                    a.scales[blk] = sc;
+            }
+        }
+        auto put = [&](long long idx, cx<T> x)
+        {
+            if constexpr (PREC != prec_full)
+            {
+                if (mid)
+                {
+                    st_elem<T, prec_f16>(a.out, idx, { ldexpf(x.re, sc), ldexpf(x.im, sc) });
+                    return;
+                }
+            }
+            st_elem<T, PREC>(a.out, idx, x);
+        };
         if (a.ns_log == 0)
         {
             // first pass: output r of butterfly j goes to j*R + r, so the workgroup's W butterflies
@@ -693,7 +780,7 @@ __global__ void __launch_bounds__((geom<T, L, ROLE, PREC>::BS)) k_pow2(args<T> a
                     const int iq        = q - tq * NH;
                     const long long o   = (long long)tq * N + h * NH + iq;
                     if (o < lim)
-                        st_elem<T, PREC>(a.out, run + o, lds[laddr<ROLE, W, G::LD>(tq, iq)]);
+                        put(run + o, lds[laddr<ROLE, W, G::LD>(tq, iq)]);
                 }
             }
         }
@@ -704,7 +791,7 @@ __global__ void __launch_bounds__((geom<T, L, ROLE, PREC>::BS)) k_pow2(args<T> a
             const long long base = (bl << (a.lo + L)) + ((col >> a.ns_log) << (a.ns_log + L)) + kt;
 #pragma unroll
             for (int k = 0; k < E; ++k)
-                st_elem<T, PREC>(a.out, base + ((long long)(t + k * TP) << a.ns_log), v[k]);
+                put(base + ((long long)(t + k * TP) << a.ns_log), v[k]);
         }
     }
     else if constexpr (ROLE == role_col)
@@ -866,6 +953,20 @@ int col_width(int L)
         return 1;
 }
 
+// butterflies per workgroup of the multi-pass kernel of radix 2^L
+template <typename T, int PREC, int LO, int HI>
+int pass_width(int L)
+{
+    if constexpr (LO <= HI)
+    {
+        if (L == LO)
+            return geom<T, LO, role_pass, PREC>::W;
+        return pass_width<T, PREC, LO + 1, HI>(L);
+    }
+    else
+        return 1;
+}
+
 // threads per transform of the one-workgroup kernel of 2^L points
 template <typename T, int LO, int HI>
 int direct_threads(int L)
@@ -902,6 +1003,143 @@ inline int log2_exact(size_t n)
     while ((size_t(1) << l) < n)
         ++l;
     return l;
+}
+
+// ---- DCT-II in one kernel (dct_plan, include/kfr/dft/fft.hpp:927-970, in Makhoul's real-transform
+// form): for n = 2N reals, z[q] = (x[4q], x[4q+2]) and z[N-1-q] = (x[4q+3], x[4q+1]) (q < N/2) is the
+// mirrored input as N complex values; Z = DFT_N(z); V[k] = A[k] + W_n^k B[k] with A, B the spectra of
+// the even and odd samples (from Z[k] and Z[N-k]); y[k] = Re(V[k] W_4n^k), y[n-k] = -Im(V[k] W_4n^k),
+// y[N] = Re((Re Z0 - Im Z0) W_4n^N). Thread t holds z[t + k TPT]: slots k < E/2 take z[q] straight
+// from one coalesced load of x[4q..4q+3], whose other half z[N-1-q] is slot E-1-k of thread TPT-1-t and
+// crosses through LDS; after the passes the partners Z[N-i] cross through LDS in two rounds of N/2.
+template <typename T>
+struct alignas(4 * sizeof(T)) quad
+{
+    T a, b, c, d;
+};
+
+template <typename T, int L>
+__global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_dct2(const T* __restrict__ x, T* __restrict__ y,
+                                                                        const cx<T>* __restrict__ tw,
+                                                                        const cx<T>* __restrict__ wn,
+                                                                        const cx<T>* __restrict__ w4n, long long count)
+{
+    using G          = geom<T, L, role_direct>;
+    using S          = typename G::S;
+    constexpr int N  = S::N;
+    constexpr int E  = S::E;
+    constexpr int TP = S::TPT;
+    constexpr int W  = G::W;
+    constexpr int HN = N / 2;
+    static_assert(!G::staged && E % 2 == 0 && G::H <= 2, "fused DCT needs whole-thread transforms");
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+    const mfma16 K{};
+    const int tid     = threadIdx.x;
+    const int tb      = tid / TP;
+    const int t       = tid % TP;
+    const long long g = (long long)blockIdx.x * W + tb;
+    const bool valid  = g < count;
+    const long long n = 2ll * N;
+    auto at           = [&](int i) -> cx<T>& { return lds[laddr<role_direct, W, G::LD>(tb, i)]; };
+    cx<T> v[E];
+#pragma unroll
+    for (int k = 0; k < E / 2; ++k)
+    {
+        quad<T> q4 = { T(0), T(0), T(0), T(0) };
+        if (valid)
+            q4 = reinterpret_cast<const quad<T>*>(x + g * n)[t + k * TP];
+        v[k]                                 = { q4.a, q4.c };
+        at((E / 2 - 1 - k) * TP + (TP - 1 - t)) = { q4.d, q4.b }; // z[N-1-q] - N/2
+    }
+    lds_barrier();
+#pragma unroll
+    for (int k = E / 2; k < E; ++k)
+        v[k] = at((k - E / 2) * TP + t);
+    lds_barrier();
+    run_pass<T, L, false, role_direct, prec_full, 0>(v, lds, tb, t, tw, K);
+    // v[k] = Z[t + k TPT]. Output pair i, n - i needs Z[i] and Z[N - i] (Z[0] for i = 0): the slots
+    // k >= E/2 (i >= N/2) find their partners among the slots k < E/2 of the workgroup's transform
+    // after round one, and those after round two, each slot emitting its outputs at once.
+    T* yb     = y + g * n;
+    auto emit = [&](int i, cx<T> a, cx<T> c)
+    {
+        if (!valid)
+            return;
+        if (i == 0)
+        {
+            // V[0] = Re Z0 + Im Z0 and V[N] = Re Z0 - Im Z0, both real
+            yb[0]          = a.re + a.im;
+            const cx<T> wN = w4n[N];
+            yb[N]          = (a.re - a.im) * wN.re;
+            return;
+        }
+        // A = (Z[i] + conj Z[N-i]) / 2, B = (Z[i] - conj Z[N-i]) / 2i, V = A + W_n^i B
+        const cx<T> A  = { T(0.5) * (a.re + c.re), T(0.5) * (a.im - c.im) };
+        const cx<T> B  = { T(0.5) * (a.im + c.im), -T(0.5) * (a.re - c.re) };
+        const cx<T> wi = wn[i];
+        const cx<T> V  = { A.re + wi.re * B.re - wi.im * B.im, A.im + wi.re * B.im + wi.im * B.re };
+        const cx<T> w  = w4n[i];
+        yb[i]          = V.re * w.re - V.im * w.im;
+        yb[n - i]      = -(V.re * w.im + V.im * w.re);
+    };
+#pragma unroll
+    for (int k = 0; k < E / 2; ++k)
+        at(t + k * TP) = v[k];
+    lds_barrier();
+#pragma unroll
+    for (int k = E / 2; k < E; ++k)
+    {
+        const int i = t + k * TP;
+        emit(i, v[k], i == HN ? v[k] : at(N - i));
+    }
+    lds_barrier();
+#pragma unroll
+    for (int k = E / 2; k < E; ++k)
+        at(t + k * TP - HN) = v[k];
+    lds_barrier();
+#pragma unroll
+    for (int k = 0; k < E / 2; ++k)
+    {
+        const int i = t + k * TP;
+        emit(i, v[k], i == 0 ? v[k] : at(N - i - HN));
+    }
+}
+
+template <typename T, int L>
+void dct2_launch(const T* x, T* y, const cx<T>* tw, const cx<T>* wn, const cx<T>* w4n, long long count, hipStream_t s)
+{
+    using G                  = geom<T, L, role_direct>;
+    static const bool lds_ok = (allow_lds(k_dct2<T, L>, G::lds_bytes), true);
+    (void)lds_ok;
+    const long long nb = (count + G::W - 1) / G::W;
+    if (nb > 0x7fffffffll)
+        throw std::runtime_error("batch too large for one launch");
+    timing::span ts([] { return timing::kernel_name("k_dct2", timing::type_name<T>(), L); },
+                    2 * (long long)sizeof(T) * (count << (L + 1)), s);
+    hipLaunchKernelGGL((k_dct2<T, L>), dim3((unsigned)nb), dim3(G::BS), G::lds_bytes, s, x, y, tw, wn, w4n, count);
+    hip_check(hipGetLastError(), "fused DCT launch");
+}
+
+// shortest length with a fused DCT: transforms of at least 16 threads (no staged short-transform loads)
+template <typename T>
+inline constexpr int dct_min_log = sizeof(T) == 4 ? 8 : 7;
+
+template <typename T, int LO, int HI>
+bool dct2_any(int L, const T* x, T* y, const cx<T>* tw, const cx<T>* wn, const cx<T>* w4n, long long count,
+              hipStream_t s)
+{
+    if constexpr (LO <= HI)
+    {
+        if (L == LO)
+        {
+            dct2_launch<T, LO>(x, y, tw, wn, w4n, count, s);
+            return true;
+        }
+        return dct2_any<T, LO + 1, HI>(L, x, y, tw, wn, w4n, count, s);
+    }
+    else
+        return false;
 }
 
 // ---- FFT convolution by overlap-save, one kernel per call.
@@ -1211,7 +1449,15 @@ pow2_fft<T>::pow2_fft(size_t n, hipStream_t s, int prec) : p(new impl)
     int mp_min = 24;
     if (const char* e = std::getenv("KFR_AMD_MULTIPASS_MIN_LOG"))
         mp_min = atoi(e);
-    if (prec == p2::prec_full && L >= mp_min)
+    // half-precision storage: the multi-pass kernels (32 butterflies per workgroup, 128-byte runs of
+    // 4-byte values) take 2^24 points, where they beat the four-step (0.94 against 1.00 ms per
+    // 256 MiB of bf16 data); from 2^17 to 2^23 the four-step's two passes are 30-50% faster than
+    // three multi-pass passes, whose kernels are bound by instructions, not bytes.
+    // KFR_AMD_HALF_MULTIPASS_MIN_LOG moves the threshold.
+    int half_mp_min = 24;
+    if (const char* e = std::getenv("KFR_AMD_HALF_MULTIPASS_MIN_LOG"))
+        half_mp_min = atoi(e);
+    if ((prec == p2::prec_full && L >= mp_min) || (prec != p2::prec_full && L >= half_mp_min))
     {
         p->multi     = true;
         const int np = (L + 7) / 8;
@@ -1251,7 +1497,8 @@ pow2_fft<T>::pow2_fft(size_t n, hipStream_t s, int prec) : p(new impl)
         size_t cap = (size_t)1 << 30; // rounds of at most 1 GiB, as the four-step below
         if (const char* e = std::getenv("KFR_AMD_FOURSTEP_CHUNK_MB"))
             cap = (size_t)std::max(1, atoi(e)) << 20;
-        p->chunk = (long long)std::max<size_t>(1, cap / (n * sizeof(cx<T>)));
+        const size_t mid = prec == p2::prec_full ? sizeof(cx<T>) : 2 * sizeof(_Float16);
+        p->chunk         = (long long)std::max<size_t>(1, cap / (n * mid));
         return;
     }
     p->four = true;
@@ -1313,6 +1560,59 @@ void pow2_fft<T>::impl::run(bool inverse, const void* in, void* out, long long b
     constexpr int ML = p2::max_log<T>;
     const size_t es  = PREC == p2::prec_full ? sizeof(cx<T>) : 2 * sizeof(_Float16); // HBM element size
     const cx<T>* t   = tw.template as<cx<T>>();
+    if constexpr (PREC != p2::prec_full)
+    {
+        if (multi)
+        {
+            // binary16 intermediates, one power-of-two scale per workgroup, ping-pong between two
+            // buffers (and two scale arrays)
+            if (w.p)
+                throw std::logic_error("half-precision transforms keep their own intermediate");
+            const long long n = 1ll << L;
+            const int np      = (int)mp_log.size();
+            const int nbuf    = np > 2 ? 2 : 1;
+            const long long c = std::min(chunk, batch);
+            scratch.ensure((size_t)c * n * es);
+            if (nbuf > 1)
+                scratch2.ensure((size_t)c * n * es);
+            long long maxwg = 1;
+            for (int q = 0; q + 1 < np; ++q)
+                maxwg = std::max(maxwg, ((c << (L - mp_log[q])) + 31) / p2::pass_width<T, PREC, 5, 8>(mp_log[q]) + 1);
+            scales.ensure((size_t)(2 * maxwg) * sizeof(int));
+            void* buf[2] = { scratch.ptr, scratch2.ptr };
+            int* scb[2]  = { scales.template as<int>(), scales.template as<int>() + maxwg };
+            for (long long b0 = 0; b0 < batch; b0 += chunk)
+            {
+                const long long nb = std::min(chunk, batch - b0);
+                int nsl = 0, prev_l = 0, prev_ns = 0, prev_w = 0;
+                for (int q = 0; q < np; ++q)
+                {
+                    const int l = mp_log[q];
+                    args<T> a;
+                    a.in        = q == 0 ? static_cast<const char*>(in) + b0 * n * es : buf[(q - 1) & 1];
+                    a.out       = q == np - 1 ? static_cast<char*>(out) + b0 * n * es : buf[q & 1];
+                    a.tw        = t + mp_tw[q];
+                    a.ta        = t + mp_ta[q];
+                    a.tb        = t + mp_tb[q];
+                    a.lbits     = mp_lbits[q];
+                    a.lo        = L - l;
+                    a.ns_log    = nsl;
+                    a.out_mid   = q + 1 < np ? 1 : 0;
+                    a.scales    = q + 1 < np ? scb[q & 1] : nullptr;
+                    a.in_scales = q > 0 ? scb[(q - 1) & 1] : nullptr;
+                    a.in_l      = prev_l;
+                    a.in_ns     = prev_ns;
+                    a.in_wlog   = prev_w;
+                    p2::launch_any<T, role_pass, PREC, 5, 8>(inverse, l, a, nb << (L - l), s);
+                    prev_l  = l;
+                    prev_ns = nsl;
+                    prev_w  = p2::ilog2c(p2::pass_width<T, PREC, 5, 8>(l));
+                    nsl += l;
+                }
+            }
+            return;
+        }
+    }
     if constexpr (PREC == p2::prec_full)
     {
         if (multi)
@@ -1444,6 +1744,19 @@ void pow2_fft<T>::run_axis(bool inverse, const void* in, void* out, int inner_lo
 }
 
 template <typename T>
+bool pow2_fft<T>::dct2(const void* x, void* y, const void* wn, const void* w4n, long long batch, hipStream_t s)
+{
+    if (p->four || p->multi || p->prec != p2::prec_full || p->L < p2::dct_min_log<T> ||
+        p->L > p2::direct_max_log<T>)
+        return false;
+    if (batch <= 0)
+        return true;
+    return p2::dct2_any<T, p2::dct_min_log<T>, p2::direct_max_log<T>>(
+        p->L, static_cast<const T*>(x), static_cast<T*>(y), p->tw.template as<cx<T>>(), static_cast<const cx<T>*>(wn),
+        static_cast<const cx<T>*>(w4n), batch, s);
+}
+
+template <typename T>
 std::string pow2_fft<T>::describe() const
 {
     const int le = p2::log_e<T>;
@@ -1462,8 +1775,10 @@ std::string pow2_fft<T>::describe() const
         std::string r;
         for (size_t q = 0; q < p->mp_log.size(); ++q)
             r += (q ? "x" : "") + std::to_string(1 << p->mp_log[q]);
-        return "pow2 multi-pass radices=" + r + " (16 butterflies per workgroup, one HBM pass each) chunk=" +
-               std::to_string(p->chunk);
+        const int w = p->prec == p2::prec_full ? 16 : 32;
+        return "pow2 multi-pass radices=" + r + " (" + std::to_string(w) +
+               " butterflies per workgroup, one HBM pass each) chunk=" + std::to_string(p->chunk) + pr +
+               (p->prec != p2::prec_full ? " intermediate=f16 scaled per workgroup" : "");
     }
     if (!p->four)
         return "pow2 radices=" + radices(p->L) + " threads/transform=" +

@@ -30,6 +30,11 @@ public:
     // transforms along a strided axis, in == out allowed: element (o, i, c) at ((o*n + i) << inner_log) + c
     // for o < outer, i < n, c < 2^inner_log (full precision, one-kernel lengths only)
     void run_axis(bool inverse, const void* in, void* out, int inner_log, long long outer, hipStream_t s);
+    // DCT-II (dct_plan, unscaled) of n = 2N reals for this plan's length N in one kernel: the mirrored
+    // input packed as N complex values on load, the N-point transform, and the real spectrum formed
+    // on store; wn holds W_n^k (k <= N) and w4n W_{4n}^k (k < n). False when this length has no fused
+    // kernel (not one workgroup, under 16 threads per transform, or half-precision storage).
+    bool dct2(const void* x, void* y, const void* wn, const void* w4n, long long batch, hipStream_t s);
     std::string describe() const;
 
 private:

@@ -3,12 +3,15 @@
     python -m kfr_amd.rocprof_summary trace  <dir>/run_results.db [--top 40]
     python -m kfr_amd.rocprof_summary counter <dir>/run_results.db FETCH_SIZE [--scale 2.0] [--match k_pow2]
     python -m kfr_amd.rocprof_summary traffic <fetch>/run_results.db <write>/run_results.db
+    python -m kfr_amd.rocprof_summary table <dir>/run_results.db MfmaUtil VALUBusy SQ_WAIT_ANY ... [--match k_pow2]
 
 `trace` lists kernels by total time (calls, total, average, share). `counter` lists, per kernel, the
 average value of one counter per dispatch (KiB as rocprofv3 reports it, and MiB after multiplying
 by --scale, the factor that makes the counter read a plain device copy of known size correctly).
 `traffic` combines a FETCH_SIZE run and a WRITE_SIZE run into HBM bytes per dispatch, with both scale
-factors taken from the calibration copy in the runs themselves.
+factors taken from the calibration copy in the runs themselves. `table` lists, per kernel, the average
+per dispatch of several counters of one run, and the share of wave cycles spent waiting
+(SQ_WAIT_ANY / SQ_WAVE_CYCLES) when both are in it.
 """
 from __future__ import annotations
 
@@ -100,6 +103,30 @@ def read_traffic(path: str, key: str) -> float | None:
     return None
 
 
+def read_table(path: str) -> list[dict]:
+    """rows of the first table written by `table` in a markdown file (column name -> cell text, the
+    kernel key under 'kernel'), or [] when the file is absent"""
+    rows: list[dict] = []
+    head: list[str] | None = None
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if not line.startswith("|"):
+                    if head is not None and rows:
+                        break
+                    continue
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if head is None:
+                    head = cells
+                elif not set(cells[0]) <= set("-:"):
+                    row = dict(zip(head, cells))
+                    row["kernel"] = row.get("kernel", "").strip("`")
+                    rows.append(row)
+    except OSError:
+        return []
+    return rows
+
+
 def trace(db: str, top: int) -> str:
     c = sqlite3.connect(db)
     rows = c.execute("select name, total_calls, total_duration, average, percentage from top_kernels "
@@ -129,6 +156,25 @@ def counter(db: str, cname: str, scale: float, match: str | None) -> str:
     return "\n".join(out)
 
 
+def table(db: str, names: list[str], match: str | None) -> str:
+    per = {n: _per_kernel(db, n) for n in names}
+    kernels: "OrderedDict[str, None]" = OrderedDict()
+    for n in names:
+        for k in per[n]:
+            if not match or match in k:
+                kernels[k] = None
+    wait = "SQ_WAIT_ANY" in names and "SQ_WAVE_CYCLES" in names
+    head = "| dispatches | " + " | ".join(names) + (" | wait share |" if wait else " |") + " kernel |"
+    out = [head, "|" + "---:|" * (len(names) + 1 + (1 if wait else 0)) + "---|"]
+    for k in kernels:
+        avg = {n: (sum(per[n][k]) / len(per[n][k]) if per[n].get(k) else float("nan")) for n in names}
+        cnt = max(len(per[n].get(k, [])) for n in names)
+        cells = " | ".join(f"{avg[n]:.4g}" for n in names)
+        extra = f" | {avg['SQ_WAIT_ANY'] / avg['SQ_WAVE_CYCLES']:.3f}" if wait and avg["SQ_WAVE_CYCLES"] else ""
+        out.append(f"| {cnt} | {cells}{extra} | `{_short(kernel_key(k))}` |")
+    return "\n".join(out)
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -145,8 +191,14 @@ def main(argv=None) -> int:
     h.add_argument("write_db")
     h.add_argument("--probe-match", default="__amd_rocclr_copyBuffer")
     h.add_argument("--probe-bytes", type=int, default=256 << 20)
+    g = sub.add_parser("table")
+    g.add_argument("db")
+    g.add_argument("names", nargs="+")
+    g.add_argument("--match", default=None)
     a = ap.parse_args(argv)
-    if a.cmd == "trace":
+    if a.cmd == "table":
+        print(table(a.db, a.names, a.match))
+    elif a.cmd == "trace":
         print(trace(a.db, a.top))
     elif a.cmd == "traffic":
         print(traffic(a.fetch_db, a.write_db, a.probe_match, a.probe_bytes))

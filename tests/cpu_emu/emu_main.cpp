@@ -4,9 +4,11 @@
 //   emu_main dct  f32|f64 n batch inverse in.bin out.bin
 //   emu_main prog f32|f64 n want inverse in.bin out.bin   (progressive units of one transform)
 //   emu_main ws   f32|f64 n batch inverse mb in.bin out.bin (execute with an mb-MiB caller workspace)
+//   emu_main half bf16|f16 n batch inverse in.bin out.bin   (half-precision storage, 4-byte values)
 // The input is copied into a buffer of exactly its size and the output buffer holds exactly the
 // output, so under AddressSanitizer every access beyond either is reported.
 #include "../../kfr_amd/csrc/fft_engine.hpp"
+#include "../../kfr_amd/csrc/fft_pow2.hpp"
 
 #include <cstdio>
 #include <cstdlib>
@@ -159,6 +161,37 @@ int main(int argc, char** argv)
         return 2;
     }
     const std::string kind = argv[1], t = argv[2];
+    if (kind == "half")
+    {
+        if (argc != 8)
+            return 2;
+        const size_t n        = std::strtoull(argv[3], nullptr, 10);
+        const long long batch = std::atoll(argv[4]);
+        const bool inverse    = std::atoi(argv[5]) != 0;
+        const std::vector<char> in = read_file(argv[6]);
+        const size_t bytes         = (size_t)batch * n * 4;
+        if (in.size() != bytes)
+            return 2;
+        char* din  = static_cast<char*>(std::malloc(bytes));
+        char* dout = static_cast<char*>(std::malloc(bytes));
+        std::memcpy(din, in.data(), bytes);
+        try
+        {
+            pow2_fft<float>(n, nullptr, t == "bf16" ? 1 : 2).run(inverse, din, dout, batch, nullptr);
+        }
+        catch (const std::exception& e)
+        {
+            std::fprintf(stderr, "error: %s\n", e.what());
+            return 3;
+        }
+        FILE* f = std::fopen(argv[7], "wb");
+        if (!f || std::fwrite(dout, 1, bytes, f) != bytes)
+            return 4;
+        std::fclose(f);
+        std::free(din);
+        std::free(dout);
+        return 0;
+    }
     if (kind != "dft" && kind != "real" && kind != "dct" && kind != "prog" && kind != "ws")
         return 2;
     return t == "f64" ? run<double>(kind, argc, argv) : run<float>(kind, argc, argv);

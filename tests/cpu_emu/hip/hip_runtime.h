@@ -6,10 +6,13 @@
 // threads that meet at __syncthreads (and at the LDS barrier builtin) on one std::barrier, and a
 // thread that returns drops out of the barrier, as an ended wave does on the GPU. The dynamic LDS of a
 // workgroup is one heap block of exactly the launch's byte count. Device memory is host memory.
-// Wave-level operations (shuffles, lane swaps, MFMA) are not emulated: the paths that use them (the
-// half-precision storage) abort here.
+// Wave-level operations (__shfl_xor, v_permlane32_swap, the 32x32x16 bf16 MFMA) exchange operands
+// through a per-wave buffer between two wave barriers; the MFMA follows the lane layout of
+// v_mfma_f32_32x32x16_bf16 (lane l, slot s: A[l%32][8(l/32)+s], B[8(l/32)+s][l%32]; result slot v:
+// D[8(v/4)+4(l/32)+v%4][l%32]) with the products summed in FP32.
 #pragma once
 
+#include <algorithm>
 #include <barrier>
 #include <condition_variable>
 #include <functional>
@@ -79,14 +82,96 @@ EMU_SMEM_IN(kfr_amd::bt)
 #define __builtin_amdgcn_s_barrier() ::emu::sync()
 #define __builtin_amdgcn_fence(...) ((void)0)
 typedef unsigned emu_u32x2 __attribute__((ext_vector_type(2)));
-#define __builtin_amdgcn_mfma_f32_32x32x16_bf16(...) (::emu::unsupported("MFMA"), f32x16{})
-#define __builtin_amdgcn_permlane32_swap(...) (::emu::unsupported("permlane32_swap"), emu_u32x2{})
+typedef __bf16 emu_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float emu_f32x16 __attribute__((ext_vector_type(16)));
 
-inline int __lane_id() { return (int)(threadIdx.x % 64); }
-template <typename V>
-inline V __shfl_xor(V, int)
+namespace emu
 {
-    emu::unsupported("__shfl_xor");
+// the lanes of one wave: a barrier of its live lanes and a slot per lane for exchanged operands
+struct wave_t
+{
+    std::barrier<> bar;
+    alignas(64) unsigned char slot[64][128];
+    explicit wave_t(std::ptrdiff_t lanes) : bar(lanes) {}
+};
+inline thread_local wave_t* cur_wave = nullptr;
+inline int lane() { return (int)(threadIdx.x % 64); }
+
+// every lane publishes `mine`, then reads lane src's
+template <typename V>
+V wave_read(const V& mine, int src)
+{
+    static_assert(sizeof(V) <= 128, "operand too large");
+    std::memcpy(cur_wave->slot[lane()], &mine, sizeof(V));
+    cur_wave->bar.arrive_and_wait();
+    V r;
+    std::memcpy(&r, cur_wave->slot[src], sizeof(V));
+    cur_wave->bar.arrive_and_wait();
+    return r;
+}
+
+inline emu_u32x2 permlane32_swap(unsigned v0, unsigned v1)
+{
+    const int l = lane();
+    struct pair
+    {
+        unsigned a, b;
+    };
+    const pair other = wave_read(pair{ v0, v1 }, l ^ 32);
+    emu_u32x2 r;
+    r[0] = l >= 32 ? other.b : v0; // the upper half of v0 takes the lower half of v1
+    r[1] = l < 32 ? other.a : v1;  // the lower half of v1 takes the upper half of v0
+    return r;
+}
+
+inline emu_f32x16 mfma_32x32x16_bf16(emu_bf16x8 a, emu_bf16x8 b, emu_f32x16 c)
+{
+    struct ops
+    {
+        float a[8], b[8];
+    };
+    ops mine;
+    for (int s = 0; s < 8; ++s)
+    {
+        mine.a[s] = (float)a[s];
+        mine.b[s] = (float)b[s];
+    }
+    std::memcpy(cur_wave->slot[lane()], &mine, sizeof(ops));
+    cur_wave->bar.arrive_and_wait();
+    float A[32][16], B[16][32];
+    for (int l = 0; l < 64; ++l)
+    {
+        ops o;
+        std::memcpy(&o, cur_wave->slot[l], sizeof(ops));
+        for (int s = 0; s < 8; ++s)
+        {
+            A[l % 32][8 * (l / 32) + s] = o.a[s];
+            B[8 * (l / 32) + s][l % 32] = o.b[s];
+        }
+    }
+    cur_wave->bar.arrive_and_wait();
+    const int l = lane();
+    emu_f32x16 d;
+    for (int v = 0; v < 16; ++v)
+    {
+        const int i = 8 * (v / 4) + 4 * (l / 32) + v % 4, j = l % 32;
+        float acc   = c[v];
+        for (int k = 0; k < 16; ++k)
+            acc += A[i][k] * B[k][j];
+        d[v] = acc;
+    }
+    return d;
+}
+} // namespace emu
+
+#define __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, B, C, ...) ::emu::mfma_32x32x16_bf16(A, B, C)
+#define __builtin_amdgcn_permlane32_swap(V0, V1, ...) ::emu::permlane32_swap(V0, V1)
+
+inline int __lane_id() { return emu::lane(); }
+template <typename V>
+inline V __shfl_xor(V v, int mask)
+{
+    return emu::wave_read(v, emu::lane() ^ mask);
 }
 inline unsigned __float_as_uint(float f)
 {
@@ -349,6 +434,9 @@ void launch(K kernel, dim3 grid, dim3 block, size_t shmem, hipStream_t, A... arg
                 if (shmem)
                     std::memset(lds.get(), 0xA5, shmem); // stale contents, as on the GPU
                 unsigned char* l = lds.get();
+                std::vector<std::unique_ptr<wave_t>> waves;
+                for (unsigned w = 0; w * 64 < nt; ++w)
+                    waves.emplace_back(new wave_t((std::ptrdiff_t)std::min(64u, nt - 64 * w)));
                 pool::get().run_block(nt,
                                       [&](unsigned t)
                                       {
@@ -358,7 +446,9 @@ void launch(K kernel, dim3 grid, dim3 block, size_t shmem, hipStream_t, A... arg
                                           blockDim  = block;
                                           gridDim   = grid;
                                           cur_lds   = l;
+                                          cur_wave  = waves[t / 64].get();
                                           kernel(args...);
+                                          cur_wave->bar.arrive_and_drop();
                                       });
             }
 }

@@ -5,8 +5,9 @@ global access outside its buffer aborts the run; the spectra are checked against
 
 Cases: the generic four-step over factors 3, 5, 7 with workgroups that straddle batch items
 (f32 at 16 points per thread), the power-of-two four-step, the mixed-radix menu lengths, Bluestein, a
-real transform and a DCT; and the f64 generic kernels built at 16 points per thread (the geometry an
-earlier build ran on the GPU, where lengths with radix-5 passes came out wrong) on those lengths.
+real transform, a DCT and the one-kernel DCT-II; half-precision storage on emulated MFMA; and the f64
+generic kernels built at 16 points per thread (the geometry an earlier build ran on the GPU, where
+lengths with radix-5 passes came out wrong) on those lengths.
 Set KFR_AMD_EMU_TSAN=1 to also run a ThreadSanitizer build (a missing barrier shows as a race)."""
 from __future__ import annotations
 
@@ -85,6 +86,60 @@ def test_emulated_real_and_dct(tmp_path):
     for inverse in (False, True):
         y = _run("asan", tmp_path, ["dct", "f32", m, 2, int(inverse)], d, np.float32).reshape(2, m)
         assert oracle.rel_rms(y, fp64_dct(d.astype(np.float64), inverse)) <= 2e-6
+
+
+# DCT-II in one kernel: f32 at 2^8 points per half (16 threads per transform, the shortest fused
+# length), f64 at 2^13 and 2^14 (the two-round LDS exchange); 128 points stays on the pack/FFT/post path
+@pytest.mark.parametrize("t,m,batch", [("f32", 512, 5), ("f32", 256, 3), ("f64", 1 << 14, 2), ("f64", 1 << 15, 1)])
+def test_emulated_fused_dct2(t, m, batch, tmp_path):
+    dt = np.float32 if t == "f32" else np.float64
+    d = oracle.gaussian(73 + m % 7, batch * m, dt).reshape(batch, m)
+    y = _run("asan", tmp_path, ["dct", t, m, batch, 0], d, dt).reshape(batch, m)
+    e = oracle.rel_rms(y, fp64_dct(d.astype(np.float64), False))
+    print(f"emulated dct-ii {t} m={m} batch={batch}: rel_rms_fp64={e:.3e}", flush=True)
+    assert e <= (1e-6 if t == "f32" else 1e-14)
+
+
+def _to_half(x: np.ndarray, storage: str) -> np.ndarray:
+    """complex values as interleaved 2-byte pairs: binary16, or bfloat16 rounded to nearest even"""
+    f = np.stack([x.real, x.imag], -1).astype(np.float32)
+    if storage == "f16":
+        return f.astype(np.float16).view(np.uint16)
+    u = f.view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def _from_half(h: np.ndarray, storage: str) -> np.ndarray:
+    if storage == "f16":
+        f = h.view(np.float16).astype(np.float64)
+    else:
+        f = (h.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+    return f[..., 0] + 1j * f[..., 1]
+
+
+# half-precision storage with the radix-16 passes on (emulated) MFMA: one workgroup (2^10), the
+# four-step (2^15) and the multi-pass kernels with their scaled binary16 intermediates (2^15 and 2^16,
+# selected through KFR_AMD_HALF_MULTIPASS_MIN_LOG)
+@pytest.mark.parametrize("storage,n,batch,multi", [("bf16", 1 << 10, 3, False), ("bf16", 1 << 15, 2, False),
+                                                   ("bf16", 1 << 15, 2, True), ("f16", 1 << 16, 1, True)])
+def test_emulated_half_storage(storage, n, batch, multi, tmp_path):
+    exe = emu_build.build("asan")
+    rng = np.random.default_rng(n + batch)
+    x = rng.standard_normal((batch, n)) + 1j * rng.standard_normal((batch, n))
+    h = _to_half(x, storage)
+    xq = _from_half(h, storage)
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    h.tofile(fin)
+    env = dict(os.environ, KFR_AMD_HALF_MULTIPASS_MIN_LOG="15" if multi else "99")
+    for inverse in (False, True):
+        r = subprocess.run([str(exe), "half", storage, str(n), str(batch), str(int(inverse)), str(fin), str(fout)],
+                           capture_output=True, text=True, timeout=900, env=env)
+        assert r.returncode == 0, r.stderr[-4000:]
+        y = _from_half(np.fromfile(fout, dtype=np.uint16).reshape(batch, n, 2), storage)
+        ref = np.fft.ifft(xq, axis=1) * n if inverse else np.fft.fft(xq, axis=1)
+        e = np.linalg.norm(y - ref) / np.linalg.norm(ref)
+        print(f"emulated {storage} n={n} multi={multi} inv={inverse}: rel_rms_fp64={e:.3e}", flush=True)
+        assert e <= (2e-3 if storage == "bf16" else 5e-4)
 
 
 @pytest.mark.skipif(os.environ.get("KFR_AMD_EMU_TSAN") != "1", reason="set KFR_AMD_EMU_TSAN=1")

@@ -1,7 +1,7 @@
 """Uneven batches in one persistent launch (kfr_amd_batch_*): every length of the mixed-length
 workload (7-smooth 18..8192 and primes 17..127, the latter through direct prime passes or Bluestein
 as the reference's planner decides) in both directions against FP64, graph replays, host pointers,
-autotuning and f64 (gpu)."""
+autotuning, every pass order and LDS padding the autotuner can choose, and f64 (gpu)."""
 from __future__ import annotations
 
 import numpy as np
@@ -67,6 +67,38 @@ def test_batch_every_length_both_directions(autotune):
     y = yd.cpu().numpy().view(np.complex64)
     worst = _check_all(y, x, sizes, inv, np.complex64)
     print(f"transforms={sizes.size} worst rel_rms={worst[0]:.3e} at n={worst[1]}", flush=True)
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+@pytest.mark.parametrize("order,padshift", [(0, 5), (0, 30), (1, 4), (2, 4), (1, 30), (2, 5)])
+def test_batch_forced_pass_orders_and_paddings(order, padshift, dtype, monkeypatch):
+    """every schedule the autotuner can choose, forced for all lengths: the planner's pass order (0),
+    its reverse (1), odd factors first (2), with LDS padding of one slot per 16 (4), per 32 (5) or none
+    (30); every length (f64: up to 1024 points) in both directions against FP64"""
+    torch = pytest.importorskip("torch")
+    monkeypatch.setenv("KFR_AMD_BATCH_ORDER", str(order))
+    monkeypatch.setenv("KFR_AMD_BATCH_PADSHIFT", str(padshift))
+    L = uneven_lengths()
+    if dtype == np.complex128:
+        L = L[L <= 1024]
+    sizes, inv, x = _workload(13 + order, 0, dtype, L)
+    b = kfr_amd.dft_batch(sizes, inv, dtype)
+    info = b.describe()
+    print(info, flush=True)
+    rdt = np.float32 if dtype == np.complex64 else np.float64
+    xd = torch.from_numpy(x.view(rdt).copy()).cuda()
+    yd = torch.full_like(xd, float("nan"))
+    b.execute(yd, xd)
+    torch.cuda.synchronize()
+    _check_all(yd.cpu().numpy().view(dtype), x, sizes, inv, dtype)
+    pads = [int(v) for v in info.split("padding=")[1].split("/")]
+    assert pads[{4: 0, 5: 1, 30: 2}[padshift]] == len(set(sizes.tolist()))
+
+
+def test_batch_rejects_unknown_padding(monkeypatch):
+    monkeypatch.setenv("KFR_AMD_BATCH_PADSHIFT", "3")
+    with pytest.raises(kfr_amd.KfrError, match="PADSHIFT"):
+        kfr_amd.dft_batch([16, 32], None, np.complex64)
 
 
 def test_batch_graph_replays_match_first_run():

@@ -62,7 +62,7 @@ def test_dct_known_answers():
     assert np.sqrt(np.mean((y - KAT["dct_inv"]) ** 2)) < 1e-5
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 8, 12, 16, 17, 64, 1000, 4096, 7680, 8000, 65536, 1 << 20])
+@pytest.mark.parametrize("n", [1, 2, 3, 8, 12, 16, 17, 64, 256, 512, 1000, 4096, 7680, 8000, 32768, 65536, 1 << 20])
 @pytest.mark.parametrize("dt", [np.float32, np.float64])
 def test_dct(n, dt):
     """lengths 4m whose half has a power-of-two or menu plan take the half-length real-transform path
@@ -75,6 +75,24 @@ def test_dct(n, dt):
         ref = fp64_dct(x.astype(np.float64), inv)
         e = oracle.rel_rms(y, ref)
         assert e <= max(2e-6 if dt == np.float32 else 1e-13, tol_kfr(n, dt)), (n, inv, e)
+
+
+@pytest.mark.parametrize("n,dt", [(512, np.float32), (2048, np.float32), (256, np.float64), (16384, np.float64)])
+def test_dct2_one_kernel_ragged_batch(n, dt):
+    """the one-kernel DCT-II (half lengths 2^8..2^15 in f32, 2^7..2^14 in f64) over a batch that leaves
+    the last workgroup part-filled; every transform against FP64"""
+    torch = pytest.importorskip("torch")
+    b = 37
+    x = oracle.gaussian(77 + n, n * b, dt)
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.full_like(xd, float("nan"))
+    p = kfr_amd.dct_plan(n, dt)
+    p.execute_batch(yd, xd, b)
+    torch.cuda.synchronize()
+    y = yd.cpu().numpy().reshape(b, n)
+    ref = fp64_dct(x.reshape(b, n).astype(np.float64))
+    e = max(oracle.rel_rms(y[i], ref[i]) for i in range(b))
+    assert e <= (1e-6 if dt == np.float32 else 1e-14), e
 
 
 def test_dct_batch_65536_x_1024_sample():
