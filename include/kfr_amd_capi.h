@@ -243,8 +243,8 @@ KFR_AMD_API void kfr_amd_conv_delete(KFR_AMD_CONV* conv);
  * hipGraph. Transform i reads and writes sizes[0] + ... + sizes[i-1] complex values into the buffers.
  * Every length must fit one workgroup (f32 up to 8192 points, f64 up to 4096; lengths with a prime
  * factor of 101 or more run Bluestein inside the same kernel and may be up to half that). autotune != 0
- * times the candidate transforms-per-workgroup of each length on the device at creation (cached per
- * process). execute: device pointers are enqueued on `stream` (NULL = default stream) without
+ * times candidate schedules of each length on the device at creation: the order of its radix passes,
+ * the LDS padding of its tile and the transforms per work item (cached per process). execute: device pointers are enqueued on `stream` (NULL = default stream) without
  * synchronising; host pointers are staged synchronously. Calls on one batch plan must not overlap.
  * dtype: 0 f32, 1 f64. No counterpart in the reference, which transforms one length per plan. */
 typedef struct KFR_AMD_BATCH { int opaque; } KFR_AMD_BATCH;

@@ -16,8 +16,9 @@ template <typename T>
 class gpu_batch
 {
 public:
-    // autotune: time the candidate transforms-per-workgroup of every distinct length on the device
-    // (results are cached per process) instead of taking the occupancy heuristic
+    // autotune: time candidate schedules of every distinct length on the device (pass order, LDS
+    // padding, transforms per work item; cached per process) instead of the planner's order, one
+    // padding slot per 16 values and the occupancy heuristic
     gpu_batch(size_t count, const size_t* sizes, const int* inverse, bool autotune);
     ~gpu_batch();
     size_t count() const;

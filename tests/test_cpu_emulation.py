@@ -8,7 +8,8 @@ Cases: the generic four-step over factors 3, 5, 7 with workgroups that straddle 
 real transform, a DCT and the one-kernel DCT-II; half-precision storage on emulated MFMA; and the f64
 generic kernels built at 16 points per thread (the geometry an earlier build ran on the GPU, where
 lengths with radix-5 passes came out wrong) on those lengths.
-Set KFR_AMD_EMU_TSAN=1 to also run a ThreadSanitizer build (a missing barrier shows as a race)."""
+Set KFR_AMD_EMU_TSAN=1 to also run a ThreadSanitizer build (a missing barrier shows as a race), and
+KFR_AMD_EMU_FULL=1 for the 16-points-per-thread f64 build and the slowest f64 length."""
 from __future__ import annotations
 
 import os
@@ -54,6 +55,12 @@ def _dft(variant, tmp_path, dtype, n, batch, inverse, seed):
 # prime pass
 F32_CASES = [(59049, 3), (30240, 3), (13440, 2), (1 << 16, 2), (3840, 3), (10007, 1), (97, 5)]
 F64_CASES = [(13440, 3), (15360, 2), (30240, 3), (62500, 1)]
+# the default run keeps the CPU suite to a few minutes: one sanitizer build, and f64 without its
+# slowest length; KFR_AMD_EMU_FULL=1 adds 15360 and the 16-points-per-thread f64 build
+FULL = os.environ.get("KFR_AMD_EMU_FULL") == "1"
+F64_VARIANTS = ["asan", "asan_pm16"] if FULL else ["asan"]
+if not FULL:
+    F64_CASES = [c for c in F64_CASES if c[0] != 15360]
 
 
 @pytest.mark.parametrize("n,batch", F32_CASES)
@@ -64,7 +71,7 @@ def test_emulated_f32_lengths(n, batch, tmp_path):
         assert e <= 1e-6
 
 
-@pytest.mark.parametrize("variant", ["asan", "asan_pm16"])
+@pytest.mark.parametrize("variant", F64_VARIANTS)
 @pytest.mark.parametrize("n,batch", F64_CASES)
 def test_emulated_f64_lengths(variant, n, batch, tmp_path):
     for inverse in (False, True):
