@@ -96,8 +96,41 @@ __global__ void __launch_bounds__(block) __attribute__((amdgpu_waves_per_eu(4)))
         const int n         = D.n;
         const cx<T>* ch     = tab + D.chirp;
 
-        // load; the inverse runs as conj(DFT(conj x)); Bluestein premultiplies by the chirp and pads
-        for (int i = c.t; i < G.n; i += G.tpt)
+        // load; the inverse runs as conj(DFT(conj x)); Bluestein premultiplies by the chirp and pads.
+        // A thread holds at most EM values of one transform (every pass gives it at most
+        // ceil(pm/R)*R <= pm + 6 of them): the loads of all of them are issued before the first is
+        // used, so their latencies overlap instead of adding up; any value beyond EM (none for the
+        // schedules the planner makes) takes the plain loop.
+        constexpr int EM = pmax_of<T> + 7;
+        {
+            cx<T> v[EM];
+#pragma unroll
+            for (int k = 0; k < EM; ++k)
+            {
+                const int i = c.t + k * G.tpt;
+                v[k]        = { T(0), T(0) };
+                if (i < n)
+                    v[k] = in[off + i];
+            }
+#pragma unroll
+            for (int k = 0; k < EM; ++k)
+            {
+                const int i = c.t + k * G.tpt;
+                if (i < G.n)
+                {
+                    cx<T> x = v[k];
+                    if (i < n)
+                    {
+                        if (inv)
+                            x = cconj(x);
+                        if (D.bluestein)
+                            x = cmul(x, ch[i]);
+                    }
+                    c.at(i) = x;
+                }
+            }
+        }
+        for (int i = c.t + EM * G.tpt; i < G.n; i += G.tpt)
         {
             cx<T> v = { T(0), T(0) };
             if (i < n)
