@@ -95,6 +95,16 @@ def test_emulated_real_and_dct(tmp_path):
         assert oracle.rel_rms(y, fp64_dct(d.astype(np.float64), inverse)) <= 2e-6
 
 
+# C2R of lengths whose half does not fit one generic workgroup: the unpacking pass into plain
+# half-length sequences, then the power-of-two kernels (2^16) or the generic four-step (30000)
+@pytest.mark.parametrize("n,batch,fmt", [(1 << 16, 2, 1), (1 << 16, 3, 0), (30000, 2, 1)])
+def test_emulated_c2r_unpack_pass(n, batch, fmt, tmp_path):
+    x = oracle.gaussian(74 + fmt, batch * n, np.float32).reshape(batch, n)
+    X = fp64_rdft(x.astype(np.float64), fmt).astype(np.complex64)
+    y = _run("asan", tmp_path, ["real", "f32", n, fmt, batch, 1], X, np.float32).reshape(batch, n)
+    assert oracle.rel_rms(y / n, x) <= 1e-6
+
+
 # DCT-II in one kernel: f32 at 2^8 points per half (16 threads per transform, the shortest fused
 # length), f64 at 2^13 and 2^14 (the two-round LDS exchange); 128 points stays on the pack/FFT/post path
 @pytest.mark.parametrize("t,m,batch", [("f32", 512, 5), ("f32", 256, 3), ("f64", 1 << 14, 2), ("f64", 1 << 15, 1)])
