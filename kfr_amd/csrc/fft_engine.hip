@@ -546,9 +546,25 @@ struct gpu_dft_real<T>::impl
 
     bool fused_forward() const { return c->algo == gpu_dft<T>::impl::CORE && !c->core->fourstep; }
 
+    // the power-of-two plan of the half length, when it has one-kernel real transforms
+    pow2_fft<T>* pow2_half() const
+    {
+        return c->algo == gpu_dft<T>::impl::CORE && c->core && c->core->pow2 ? c->core->pow2.get() : nullptr;
+    }
+    static bool plain_ld(const xload<T>& l, long long dist)
+    {
+        return l.mode == ld_plain && l.src == 0 && l.q == nullptr && l.mtab == nullptr && l.lim == LLONG_MAX &&
+               l.pdist == dist;
+    }
+
     void forward_op(const xload<T>& ld, cx<T>* out, long long odist, long long batch, hipStream_t s)
     {
         if (batch <= 0)
+            return;
+        // power-of-two halves in one workgroup: transform and repack in one k_pow2-style kernel
+        if (pow2_fft<T>* h = pow2_half(); h && plain_ld(ld, (long long)csize) &&
+            h->real(false, static_cast<const cx<T>*>(ld.p) + ld.b0 * (long long)csize, (long long)csize, out, odist,
+                    rtw.ptr, fmt, batch, s))
             return;
         if (fused_forward())
         {
@@ -604,12 +620,17 @@ struct gpu_dft_real<T>::impl
     // inverse: `src` describes the half spectrum (plain or a product of two spectra)
     void inverse_op(xload<T> src, const xstore<T>& st, long long batch, hipStream_t s)
     {
+        if (batch <= 0)
+            return;
+        if (pow2_fft<T>* h = pow2_half(); h && plain_ld(src, (long long)complex_size()) && st.mode == st_plain &&
+            st.pdist == (long long)csize && st.mtab == nullptr && st.lim == LLONG_MAX &&
+            h->real(true, static_cast<const cx<T>*>(src.p) + src.b0 * src.pdist, src.pdist,
+                    static_cast<cx<T>*>(st.p) + st.b0 * (long long)csize, (long long)csize, rtw.ptr, fmt, batch, s))
+            return;
         src.mode = ld_c2r;
         src.r    = rtw.as<cx<T>>();
         src.n    = (long long)csize;
         src.flag = fmt;
-        if (batch <= 0)
-            return;
         if (fused_forward())
         {
             // one workgroup per transform: the unpacking runs in the load phase of the same kernel

@@ -1142,6 +1142,226 @@ bool dct2_any(int L, const T* x, T* y, const cx<T>* tw, const cx<T>* wn, const c
         return false;
 }
 
+// ---- Real transforms in one kernel (dft_plan_real, to_fmt / from_fmt of src/dft/fft-impl.hpp:1799-1896)
+// for half lengths N = 2^L that fit one workgroup. The n = 2N reals are N complex values z.
+// R2C: Z = DFT_N(z) in registers (slot k of thread t holds Z[t + k TPT]); the partner Z[N - i] of
+// every slot crosses through LDS in two rounds of N/2 values, and each slot writes its own output
+//   X[i] = (F1 + F2 w_i) / 2                     for 0 < i < N/2
+//   X[i] = conj((F1' - F2' w_{N-i}) / 2)         for N/2 < i < N, primes for the pair (N - i, i)
+// with F1 = Z[i] + conj Z[N-i], F2 = Z[i] - conj Z[N-i], w_k = rtw[k] = exp(-i(2 pi k/n + pi/2));
+// X[0] and X[N] (CCs) or X[0] packed (Perm) from Z[0], X[N/2] = conj Z[N/2].
+// C2R: each slot forms its packed input from X[i] and X[N - i] (two coalesced loads, the second in
+// reverse order), the inverse transform runs, and the reals are stored as plain pairs.
+template <typename T, int L>
+__global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_rfft_fwd(const cx<T>* __restrict__ x,
+                                                                            cx<T>* __restrict__ out, long long odist,
+                                                                            const cx<T>* __restrict__ tw,
+                                                                            const cx<T>* __restrict__ rtw, int fmt,
+                                                                            long long count)
+{
+    using G          = geom<T, L, role_direct>;
+    using S          = typename G::S;
+    constexpr int N  = S::N;
+    constexpr int E  = S::E;
+    constexpr int TP = S::TPT;
+    constexpr int W  = G::W;
+    constexpr int HN = N / 2;
+    static_assert(!G::staged && E % 2 == 0 && G::H <= 2, "fused real transform needs whole-thread transforms");
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+    const mfma16 K{};
+    const int tid     = threadIdx.x;
+    const int tb      = tid / TP;
+    const int t       = tid % TP;
+    const long long g = (long long)blockIdx.x * W + tb;
+    const bool valid  = g < count;
+    auto at           = [&](int i) -> cx<T>& { return lds[laddr<role_direct, W, G::LD>(tb, i)]; };
+    cx<T> v[E];
+    const cx<T>* xb = x + g * N + t;
+#pragma unroll
+    for (int k = 0; k < E; ++k)
+        v[k] = valid ? xb[k * TP] : cx<T>{ T(0), T(0) };
+    run_pass<T, L, false, role_direct, prec_full, 0>(v, lds, tb, t, tw, K);
+    cx<T>* o  = out + g * odist;
+    auto emit = [&](int i, cx<T> a, cx<T> c)
+    {
+        if (!valid)
+            return;
+        if (i == 0)
+        {
+            if (fmt == 1)
+            {
+                o[0] = { a.re + a.im, T(0) };
+                o[N] = { a.re - a.im, T(0) };
+            }
+            else
+                o[0] = { a.re + a.im, a.re - a.im };
+            return;
+        }
+        if (i == HN)
+        {
+            o[HN] = { a.re, -a.im };
+            return;
+        }
+        if (i < HN)
+        {
+            const cx<T> f1 = { a.re + c.re, a.im - c.im }; // a + conj c
+            const cx<T> f2 = { a.re - c.re, a.im + c.im }; // a - conj c
+            const cx<T> tt = cmul(f2, rtw[i]);
+            o[i]           = { T(0.5) * (f1.re + tt.re), T(0.5) * (f1.im + tt.im) };
+        }
+        else
+        {
+            // pair (k, N - k) with k = N - i: Z[k] = c, Z[N - k] = a
+            const cx<T> f1 = { c.re + a.re, c.im - a.im };
+            const cx<T> f2 = { c.re - a.re, c.im + a.im };
+            const cx<T> tt = cmul(f2, rtw[N - i]);
+            o[i]           = { T(0.5) * (f1.re - tt.re), -T(0.5) * (f1.im - tt.im) };
+        }
+    };
+#pragma unroll
+    for (int k = 0; k < E / 2; ++k)
+        at(t + k * TP) = v[k];
+    lds_barrier();
+#pragma unroll
+    for (int k = E / 2; k < E; ++k)
+    {
+        const int i = t + k * TP;
+        emit(i, v[k], i == HN ? v[k] : at(N - i));
+    }
+    lds_barrier();
+#pragma unroll
+    for (int k = E / 2; k < E; ++k)
+        at(t + k * TP - HN) = v[k];
+    lds_barrier();
+#pragma unroll
+    for (int k = 0; k < E / 2; ++k)
+    {
+        const int i = t + k * TP;
+        emit(i, v[k], i == 0 ? v[k] : at(N - i - HN));
+    }
+}
+
+template <typename T, int L>
+__global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_rfft_inv(const cx<T>* __restrict__ spec,
+                                                                            long long idist, cx<T>* __restrict__ y,
+                                                                            const cx<T>* __restrict__ tw,
+                                                                            const cx<T>* __restrict__ rtw, int fmt,
+                                                                            long long count)
+{
+    using G          = geom<T, L, role_direct>;
+    using S          = typename G::S;
+    constexpr int N  = S::N;
+    constexpr int E  = S::E;
+    constexpr int TP = S::TPT;
+    constexpr int W  = G::W;
+    static_assert(!G::staged, "fused real transform needs whole-thread transforms");
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+    const mfma16 K{};
+    const int tid     = threadIdx.x;
+    const int tb      = tid / TP;
+    const int t       = tid % TP;
+    const long long g = (long long)blockIdx.x * W + tb;
+    const bool valid  = g < count;
+    const cx<T>* X    = spec + g * idist;
+    cx<T> v[E], u[E];
+    // X[i] and X[N - i] of every slot (X[N] is the Nyquist value in CCs; i = 0 reads X[0] twice)
+#pragma unroll
+    for (int k = 0; k < E; ++k)
+    {
+        const int i = t + k * TP;
+        v[k]        = { T(0), T(0) };
+        u[k]        = { T(0), T(0) };
+        if (valid)
+        {
+            v[k] = X[i];
+            u[k] = X[i == 0 ? (fmt == 1 ? N : 0) : N - i];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < E; ++k)
+    {
+        const int i    = t + k * TP;
+        const cx<T> xk = v[k], xn = u[k];
+        if (i == 0)
+            v[k] = fmt == 1 ? cx<T>{ xk.re + xn.re, xk.re - xn.re } : cx<T>{ xk.re + xk.im, xk.re - xk.im };
+        else if (2 * i == N)
+            v[k] = { T(2) * xk.re, T(-2) * xk.im };
+        else if (2 * i < N)
+        {
+            const cx<T> f1 = { xk.re + xn.re, xk.im - xn.im };
+            const cx<T> f2 = { xk.re - xn.re, xk.im + xn.im };
+            const cx<T> tt = cmulc(f2, rtw[i]);
+            v[k]           = { f1.re + tt.re, f1.im + tt.im };
+        }
+        else
+        {
+            // pair (k', N - k') with k' = N - i: X[k'] = xn, X[N - k'] = xk
+            const cx<T> f1 = { xn.re + xk.re, xn.im - xk.im };
+            const cx<T> f2 = { xn.re - xk.re, xn.im + xk.im };
+            const cx<T> tt = cmulc(f2, rtw[N - i]);
+            v[k]           = { f1.re - tt.re, -(f1.im - tt.im) };
+        }
+    }
+    run_pass<T, L, true, role_direct, prec_full, 0>(v, lds, tb, t, tw, K);
+    if (valid)
+    {
+        cx<T>* yb = y + g * N + t;
+#pragma unroll
+        for (int k = 0; k < E; ++k)
+            yb[k * TP] = v[k];
+    }
+}
+
+template <typename T, int L, bool INV>
+void rfft_launch(const cx<T>* in, long long idist, cx<T>* out, long long odist, const cx<T>* tw, const cx<T>* rtw,
+                 int fmt, long long count, hipStream_t s)
+{
+    using G = geom<T, L, role_direct>;
+    if constexpr (INV)
+    {
+        static const bool lds_ok = (allow_lds(k_rfft_inv<T, L>, G::lds_bytes), true);
+        (void)lds_ok;
+    }
+    else
+    {
+        static const bool lds_ok = (allow_lds(k_rfft_fwd<T, L>, G::lds_bytes), true);
+        (void)lds_ok;
+    }
+    const long long nb = (count + G::W - 1) / G::W;
+    if (nb > 0x7fffffffll)
+        throw std::runtime_error("batch too large for one launch");
+    // algorithmic bytes: the n = 2N reals one way, N + 1 (CCs) or N (Perm) complex values the other
+    const long long bytes = (long long)sizeof(cx<T>) * count * ((2ll << L) + (fmt == 1 ? 1 : 0));
+    timing::span ts([] { return timing::kernel_name(INV ? "k_rfft_inv" : "k_rfft_fwd", timing::type_name<T>(), L); },
+                    bytes, s);
+    if constexpr (INV)
+        hipLaunchKernelGGL((k_rfft_inv<T, L>), dim3((unsigned)nb), dim3(G::BS), G::lds_bytes, s, in, idist, out, tw,
+                           rtw, fmt, count);
+    else
+        hipLaunchKernelGGL((k_rfft_fwd<T, L>), dim3((unsigned)nb), dim3(G::BS), G::lds_bytes, s, in, out, odist, tw,
+                           rtw, fmt, count);
+    hip_check(hipGetLastError(), "fused real transform launch");
+}
+
+template <typename T, bool INV, int LO, int HI>
+bool rfft_any(int L, const cx<T>* in, long long idist, cx<T>* out, long long odist, const cx<T>* tw,
+              const cx<T>* rtw, int fmt, long long count, hipStream_t s)
+{
+    if constexpr (LO <= HI)
+    {
+        if (L == LO)
+        {
+            rfft_launch<T, LO, INV>(in, idist, out, odist, tw, rtw, fmt, count, s);
+            return true;
+        }
+        return rfft_any<T, INV, LO + 1, HI>(L, in, idist, out, odist, tw, rtw, fmt, count, s);
+    }
+    else
+        return false;
+}
+
 // ---- FFT convolution by overlap-save, one kernel per call.
 // Output m of the linear convolution y = u * f (|f| = nf) needs u[m - nf + 1 .. m]. A segment of
 // lv = N - nf + 1 consecutive outputs therefore reads N consecutive inputs z[n] = u[i0 + n], and the
@@ -1754,6 +1974,26 @@ bool pow2_fft<T>::dct2(const void* x, void* y, const void* wn, const void* w4n, 
     return p2::dct2_any<T, p2::dct_min_log<T>, p2::direct_max_log<T>>(
         p->L, static_cast<const T*>(x), static_cast<T*>(y), p->tw.template as<cx<T>>(), static_cast<const cx<T>*>(wn),
         static_cast<const cx<T>*>(w4n), batch, s);
+}
+
+template <typename T>
+bool pow2_fft<T>::real(bool inverse, const void* in, long long idist, void* out, long long odist, const void* rtw,
+                       int fmt, long long batch, hipStream_t s)
+{
+    if (p->four || p->multi || p->prec != p2::prec_full || p->L < p2::dct_min_log<T> ||
+        p->L > p2::direct_max_log<T>)
+        return false;
+    if (batch <= 0)
+        return true;
+    const cx<T>* t = p->tw.template as<cx<T>>();
+    const auto* i  = static_cast<const cx<T>*>(in);
+    auto* o        = static_cast<cx<T>*>(out);
+    const auto* r  = static_cast<const cx<T>*>(rtw);
+    if (inverse)
+        return p2::rfft_any<T, true, p2::dct_min_log<T>, p2::direct_max_log<T>>(p->L, i, idist, o, odist, t, r, fmt,
+                                                                                 batch, s);
+    return p2::rfft_any<T, false, p2::dct_min_log<T>, p2::direct_max_log<T>>(p->L, i, idist, o, odist, t, r, fmt, batch,
+                                                                              s);
 }
 
 template <typename T>

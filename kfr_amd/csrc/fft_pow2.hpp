@@ -35,6 +35,12 @@ public:
     // on store; wn holds W_n^k (k <= N) and w4n W_{4n}^k (k < n). False when this length has no fused
     // kernel (not one workgroup, under 16 threads per transform, or half-precision storage).
     bool dct2(const void* x, void* y, const void* wn, const void* w4n, long long batch, hipStream_t s);
+    // real transform (dft_plan_real) of n = 2N reals for this plan's length N in one kernel: forward
+    // reads the reals as N complex values and writes the CCs (fmt 1, N + 1 values) or Perm (fmt 0)
+    // spectrum at distance odist; inverse reads spectra at distance idist and writes the reals. rtw
+    // holds exp(-i(2 pi k/n + pi/2)) for k <= N/2. False when this length has no fused kernel.
+    bool real(bool inverse, const void* in, long long idist, void* out, long long odist, const void* rtw, int fmt,
+              long long batch, hipStream_t s);
     std::string describe() const;
 
 private:

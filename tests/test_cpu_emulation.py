@@ -95,6 +95,22 @@ def test_emulated_real_and_dct(tmp_path):
         assert oracle.rel_rms(y, fp64_dct(d.astype(np.float64), inverse)) <= 2e-6
 
 
+# R2C and C2R in one kernel for power-of-two halves that fit one workgroup (f32 2^8 and 2^15 halves,
+# the two-round partner exchange at 2^15; f64 2^14), both pack formats
+@pytest.mark.parametrize("t,n,batch", [("f32", 512, 5), ("f32", 1 << 16, 2), ("f64", 1 << 15, 2)])
+def test_emulated_real_one_kernel(t, n, batch, tmp_path):
+    dt = np.float32 if t == "f32" else np.float64
+    cdt = np.complex64 if t == "f32" else np.complex128
+    bound = 1e-6 if t == "f32" else 1e-14
+    for fmt in (0, 1):
+        x = oracle.gaussian(75 + fmt, batch * n, dt).reshape(batch, n)
+        X = _run("asan", tmp_path, ["real", t, n, fmt, batch, 0], x, cdt).reshape(batch, -1)
+        ref = fp64_rdft(x.astype(np.float64), fmt)
+        assert oracle.rel_rms(X, ref) <= bound
+        y = _run("asan", tmp_path, ["real", t, n, fmt, batch, 1], ref.astype(cdt), dt).reshape(batch, n)
+        assert oracle.rel_rms(y / n, x) <= bound
+
+
 # C2R of lengths whose half does not fit one generic workgroup: the unpacking pass into plain
 # half-length sequences, then the power-of-two kernels (2^16) or the generic four-step (30000)
 @pytest.mark.parametrize("n,batch,fmt", [(1 << 16, 2, 1), (1 << 16, 3, 0), (30000, 2, 1)])
