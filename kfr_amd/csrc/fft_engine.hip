@@ -900,7 +900,7 @@ struct gpu_dct<T>::impl
     dev_mem wn; // W_n^k, k <= n/2
     dev_mem z;  // packed n/2-point data of one round
     long long chunk = 1;
-    bool fused      = false; // DCT-II in one kernel (power-of-two n/2 within one workgroup)
+    bool fused      = false; // DCT-II and DCT-III in one kernel (power-of-two n/2 within one workgroup)
     explicit impl(size_t len)
     {
         n             = len;
@@ -914,7 +914,7 @@ struct gpu_dct<T>::impl
             wn.reset((h + 1) * sizeof(cx<T>));
             gen_twiddles(wn.as<cx<T>>(), (long long)h + 1, (long long)n, 1, 0, tw_linear, 0, s);
             chunk = (long long)std::max<size_t>(1, ((size_t)512 << 20) / (h * sizeof(cx<T>)));
-            fused = ch->core && ch->core->pow2 && ch->core->pow2->dct2(nullptr, nullptr, nullptr, nullptr, 0, s);
+            fused = ch->core && ch->core->pow2 && ch->core->pow2->dct(false, nullptr, nullptr, nullptr, nullptr, 0, s);
         }
         else
             c.reset(new typename gpu_dft<T>::impl(n));
@@ -946,9 +946,9 @@ void gpu_dct<T>::execute(void* out, const void* in, long long batch, bool invers
     {
         impl& q           = *p;
         const long long h = n / 2;
-        if (!inverse && q.fused)
+        if (q.fused)
         {
-            q.ch->core->pow2->dct2(in, out, q.wn.ptr, q.w.ptr, batch, s);
+            q.ch->core->pow2->dct(inverse, in, out, q.wn.ptr, q.w.ptr, batch, s);
             return;
         }
         q.z.ensure((size_t)(std::min(batch, q.chunk) * h) * sizeof(cx<T>));

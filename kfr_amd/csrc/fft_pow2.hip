@@ -1106,6 +1106,133 @@ __global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_dct2(const T*
     }
 }
 
+// DCT-III (dct_plan's inverse, unscaled) in one kernel, the mirror image of k_dct2: with V[k] = y'[k] W_4n^k
+// (y'[0] = y[0]/2) and H[k] = (conj V[k] + V[n-k])/2, slot i forms
+// Z'[i] = (H[i] + conj H[N-i]) + i (H[i] - conj H[N-i]) conj(W_n^i), the inverse N-point transform runs, and
+// x[4q..4q+3] = (u[q].re, u[N-1-q].im, u[q].im, u[N-1-q].re) leaves as one 16-byte store per q < N/2,
+// u[N-1-q] crossing through LDS from slot E-1-k of thread TPT-1-t.
+template <typename T, int L>
+__global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_dct3(const T* __restrict__ y, T* __restrict__ x,
+                                                                        const cx<T>* __restrict__ tw,
+                                                                        const cx<T>* __restrict__ wn,
+                                                                        const cx<T>* __restrict__ w4n, long long count)
+{
+    using G          = geom<T, L, role_direct>;
+    using S          = typename G::S;
+    constexpr int N  = S::N;
+    constexpr int E  = S::E;
+    constexpr int TP = S::TPT;
+    constexpr int W  = G::W;
+    constexpr int HN = N / 2;
+    static_assert(!G::staged && E % 2 == 0 && G::H <= 2, "fused DCT needs whole-thread transforms");
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+    const mfma16 K{};
+    const int tid     = threadIdx.x;
+    const int tb      = tid / TP;
+    const int t       = tid % TP;
+    const long long g = (long long)blockIdx.x * W + tb;
+    const bool valid  = g < count;
+    constexpr long long n = 2ll * N;
+    auto at               = [&](int i) -> cx<T>& { return lds[laddr<role_direct, W, G::LD>(tb, i)]; };
+    const T* yb           = y + g * n;
+    // H[kk] = (conj V[kk] + V[n-kk]) / 2 from a = y[kk], b = y[n-kk] and w = W_4n^kk, using
+    // W_4n^(n-kk) = (-w.im, -w.re) (the table values swapped, so exactly the rounded table entry);
+    // kk = 0 reads y[0] for both (V[n] = V[0]) and halves it
+    auto Hs = [&](bool zero, T a, T b, cx<T> w) -> cx<T>
+    {
+        if (zero)
+            return { T(0.5) * a * w.re, T(0) };
+        const cx<T> w1 = { -w.im, -w.re };
+        return { T(0.5) * (a * w.re + b * w1.re), T(0.5) * (b * w1.im - a * w.im) };
+    };
+    // slot i holds H[i] (two coalesced loads, one in reverse order); the partner H[N - i] crosses
+    // through LDS in two rounds of N/2 values as in k_dct2, H[N] (the partner of slot 0) is computed
+    // by every thread from y[N]
+    auto pack = [&](int i, cx<T> a, cx<T> c) -> cx<T>
+    {
+        // (a + conj c) + i (a - conj c) conj(W_n^i)
+        const cx<T> sm = { a.re + c.re, a.im - c.im };
+        const cx<T> d  = { a.re - c.re, a.im + c.im };
+        const cx<T> w  = wn[i];
+        const cx<T> e  = { d.re * w.re + d.im * w.im, d.im * w.re - d.re * w.im };
+        return { sm.re - e.im, sm.im + e.re };
+    };
+    cx<T> v[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k)
+    {
+        const int i = t + k * TP;
+        T ya = T(0), yr = T(0);
+        if (valid)
+        {
+            ya = yb[i];
+            yr = yb[i == 0 ? 0 : n - i];
+        }
+        v[k] = Hs(i == 0, ya, yr, w4n[i]);
+    }
+    const T yN      = valid ? yb[N] : T(0);
+    const cx<T> hN  = Hs(false, yN, yN, w4n[N]);
+    cx<T> zu[E / 2];
+#pragma unroll
+    for (int k = 0; k < E / 2; ++k)
+        at(t + k * TP) = v[k];
+    lds_barrier();
+#pragma unroll
+    for (int k = E / 2; k < E; ++k)
+    {
+        const int i   = t + k * TP;
+        zu[k - E / 2] = pack(i, v[k], i == HN ? v[k] : at(N - i));
+    }
+    lds_barrier();
+#pragma unroll
+    for (int k = E / 2; k < E; ++k)
+        at(t + k * TP - HN) = v[k];
+    lds_barrier();
+#pragma unroll
+    for (int k = 0; k < E / 2; ++k)
+    {
+        const int i = t + k * TP;
+        v[k]        = pack(i, v[k], i == 0 ? hN : at(N - i - HN));
+    }
+#pragma unroll
+    for (int k = E / 2; k < E; ++k)
+        v[k] = zu[k - E / 2];
+    lds_barrier();
+    run_pass<T, L, true, role_direct, prec_full, 0>(v, lds, tb, t, tw, K);
+    // v[k] = u[t + k TPT]; the upper half of the slots crosses to the lower half in mirror order
+#pragma unroll
+    for (int k = E / 2; k < E; ++k)
+        at(t + k * TP - HN) = v[k];
+    lds_barrier();
+    if (!valid)
+        return;
+#pragma unroll
+    for (int k = 0; k < E / 2; ++k)
+    {
+        const int q    = t + k * TP;
+        const cx<T> uq = v[k];
+        const cx<T> um = at(HN - 1 - q); // u[N - 1 - q]
+        quad<T> o4     = { uq.re, um.im, uq.im, um.re };
+        reinterpret_cast<quad<T>*>(x + g * n)[q] = o4;
+    }
+}
+
+template <typename T, int L>
+void dct3_launch(const T* y, T* x, const cx<T>* tw, const cx<T>* wn, const cx<T>* w4n, long long count, hipStream_t s)
+{
+    using G                  = geom<T, L, role_direct>;
+    static const bool lds_ok = (allow_lds(k_dct3<T, L>, G::lds_bytes), true);
+    (void)lds_ok;
+    const long long nb = (count + G::W - 1) / G::W;
+    if (nb > 0x7fffffffll)
+        throw std::runtime_error("batch too large for one launch");
+    timing::span ts([] { return timing::kernel_name("k_dct3", timing::type_name<T>(), L); },
+                    2 * (long long)sizeof(T) * (count << (L + 1)), s);
+    hipLaunchKernelGGL((k_dct3<T, L>), dim3((unsigned)nb), dim3(G::BS), G::lds_bytes, s, y, x, tw, wn, w4n, count);
+    hip_check(hipGetLastError(), "fused DCT-III launch");
+}
+
 template <typename T, int L>
 void dct2_launch(const T* x, T* y, const cx<T>* tw, const cx<T>* wn, const cx<T>* w4n, long long count, hipStream_t s)
 {
@@ -1126,17 +1253,20 @@ template <typename T>
 inline constexpr int dct_min_log = sizeof(T) == 4 ? 8 : 7;
 
 template <typename T, int LO, int HI>
-bool dct2_any(int L, const T* x, T* y, const cx<T>* tw, const cx<T>* wn, const cx<T>* w4n, long long count,
-              hipStream_t s)
+bool dct2_any(bool inverse, int L, const T* x, T* y, const cx<T>* tw, const cx<T>* wn, const cx<T>* w4n,
+              long long count, hipStream_t s)
 {
     if constexpr (LO <= HI)
     {
         if (L == LO)
         {
-            dct2_launch<T, LO>(x, y, tw, wn, w4n, count, s);
+            if (inverse)
+                dct3_launch<T, LO>(x, y, tw, wn, w4n, count, s);
+            else
+                dct2_launch<T, LO>(x, y, tw, wn, w4n, count, s);
             return true;
         }
-        return dct2_any<T, LO + 1, HI>(L, x, y, tw, wn, w4n, count, s);
+        return dct2_any<T, LO + 1, HI>(inverse, L, x, y, tw, wn, w4n, count, s);
     }
     else
         return false;
@@ -1964,7 +2094,8 @@ void pow2_fft<T>::run_axis(bool inverse, const void* in, void* out, int inner_lo
 }
 
 template <typename T>
-bool pow2_fft<T>::dct2(const void* x, void* y, const void* wn, const void* w4n, long long batch, hipStream_t s)
+bool pow2_fft<T>::dct(bool inverse, const void* x, void* y, const void* wn, const void* w4n, long long batch,
+                      hipStream_t s)
 {
     if (p->four || p->multi || p->prec != p2::prec_full || p->L < p2::dct_min_log<T> ||
         p->L > p2::direct_max_log<T>)
@@ -1972,8 +2103,8 @@ bool pow2_fft<T>::dct2(const void* x, void* y, const void* wn, const void* w4n, 
     if (batch <= 0)
         return true;
     return p2::dct2_any<T, p2::dct_min_log<T>, p2::direct_max_log<T>>(
-        p->L, static_cast<const T*>(x), static_cast<T*>(y), p->tw.template as<cx<T>>(), static_cast<const cx<T>*>(wn),
-        static_cast<const cx<T>*>(w4n), batch, s);
+        inverse, p->L, static_cast<const T*>(x), static_cast<T*>(y), p->tw.template as<cx<T>>(),
+        static_cast<const cx<T>*>(wn), static_cast<const cx<T>*>(w4n), batch, s);
 }
 
 template <typename T>

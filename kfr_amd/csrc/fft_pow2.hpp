@@ -32,9 +32,10 @@ public:
     void run_axis(bool inverse, const void* in, void* out, int inner_log, long long outer, hipStream_t s);
     // DCT-II (dct_plan, unscaled) of n = 2N reals for this plan's length N in one kernel: the mirrored
     // input packed as N complex values on load, the N-point transform, and the real spectrum formed
-    // on store; wn holds W_n^k (k <= N) and w4n W_{4n}^k (k < n). False when this length has no fused
-    // kernel (not one workgroup, under 16 threads per transform, or half-precision storage).
-    bool dct2(const void* x, void* y, const void* wn, const void* w4n, long long batch, hipStream_t s);
+    // on store; inverse: DCT-III (dct_plan's inverse, unscaled), the same steps backwards. wn holds
+    // W_n^k (k <= N) and w4n W_{4n}^k (k < n). False when this length has no fused kernel (not one
+    // workgroup, under 16 threads per transform, or half-precision storage).
+    bool dct(bool inverse, const void* x, void* y, const void* wn, const void* w4n, long long batch, hipStream_t s);
     // real transform (dft_plan_real) of n = 2N reals for this plan's length N in one kernel: forward
     // reads the reals as N complex values and writes the CCs (fmt 1, N + 1 values) or Perm (fmt 0)
     // spectrum at distance odist; inverse reads spectra at distance idist and writes the reals. rtw

@@ -5,7 +5,7 @@ global access outside its buffer aborts the run; the spectra are checked against
 
 Cases: the generic four-step over factors 3, 5, 7 with workgroups that straddle batch items
 (f32 at 16 points per thread), the power-of-two four-step, the mixed-radix menu lengths, Bluestein, a
-real transform, a DCT and the one-kernel DCT-II; half-precision storage on emulated MFMA; and the f64
+real transform, a DCT, the one-kernel DCT-II/III and R2C/C2R; half-precision storage on emulated MFMA; and the f64
 generic kernels built at 16 points per thread (the geometry an earlier build ran on the GPU, where
 lengths with radix-5 passes came out wrong) on those lengths.
 Set KFR_AMD_EMU_TSAN=1 to also run a ThreadSanitizer build (a missing barrier shows as a race), and
@@ -121,16 +121,19 @@ def test_emulated_c2r_unpack_pass(n, batch, fmt, tmp_path):
     assert oracle.rel_rms(y / n, x) <= 1e-6
 
 
-# DCT-II in one kernel: f32 at 2^8 points per half (16 threads per transform, the shortest fused
-# length), f64 at 2^13 and 2^14 (the two-round LDS exchange); 128 points stays on the pack/FFT/post path
-@pytest.mark.parametrize("t,m,batch", [("f32", 512, 5), ("f32", 256, 3), ("f64", 1 << 14, 2), ("f64", 1 << 15, 1)])
-def test_emulated_fused_dct2(t, m, batch, tmp_path):
+# DCT-II and DCT-III in one kernel: f32 at 2^8 points per half (16 threads per transform, the shortest
+# fused length) and 2^15 (two-round LDS exchange), f64 at 2^13 and 2^14; 128 points f32 stays on the
+# pack/FFT/post path
+@pytest.mark.parametrize("t,m,batch", [("f32", 512, 5), ("f32", 256, 3), ("f32", 1 << 16, 1), ("f64", 1 << 14, 2),
+                                       ("f64", 1 << 15, 1)])
+def test_emulated_fused_dct(t, m, batch, tmp_path):
     dt = np.float32 if t == "f32" else np.float64
     d = oracle.gaussian(73 + m % 7, batch * m, dt).reshape(batch, m)
-    y = _run("asan", tmp_path, ["dct", t, m, batch, 0], d, dt).reshape(batch, m)
-    e = oracle.rel_rms(y, fp64_dct(d.astype(np.float64), False))
-    print(f"emulated dct-ii {t} m={m} batch={batch}: rel_rms_fp64={e:.3e}", flush=True)
-    assert e <= (1e-6 if t == "f32" else 1e-14)
+    for inverse in (False, True):
+        y = _run("asan", tmp_path, ["dct", t, m, batch, int(inverse)], d, dt).reshape(batch, m)
+        e = oracle.rel_rms(y, fp64_dct(d.astype(np.float64), inverse))
+        print(f"emulated dct {t} m={m} batch={batch} inverse={inverse}: rel_rms_fp64={e:.3e}", flush=True)
+        assert e <= (1e-6 if t == "f32" else 1e-14)
 
 
 def _to_half(x: np.ndarray, storage: str) -> np.ndarray:

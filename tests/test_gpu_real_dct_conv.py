@@ -79,20 +79,21 @@ def test_dct(n, dt):
 
 @pytest.mark.parametrize("n,dt", [(512, np.float32), (2048, np.float32), (256, np.float64), (16384, np.float64)])
 def test_dct2_one_kernel_ragged_batch(n, dt):
-    """the one-kernel DCT-II (half lengths 2^8..2^15 in f32, 2^7..2^14 in f64) over a batch that leaves
-    the last workgroup part-filled; every transform against FP64"""
+    """the one-kernel DCT-II and DCT-III (half lengths 2^8..2^15 in f32, 2^7..2^14 in f64) over a batch
+    that leaves the last workgroup part-filled; every transform against FP64"""
     torch = pytest.importorskip("torch")
     b = 37
     x = oracle.gaussian(77 + n, n * b, dt)
     xd = torch.from_numpy(x).cuda()
-    yd = torch.full_like(xd, float("nan"))
     p = kfr_amd.dct_plan(n, dt)
-    p.execute_batch(yd, xd, b)
-    torch.cuda.synchronize()
-    y = yd.cpu().numpy().reshape(b, n)
-    ref = fp64_dct(x.reshape(b, n).astype(np.float64))
-    e = max(oracle.rel_rms(y[i], ref[i]) for i in range(b))
-    assert e <= (1e-6 if dt == np.float32 else 1e-14), e
+    for inverse in (False, True):
+        yd = torch.full_like(xd, float("nan"))
+        p.execute_batch(yd, xd, b, inverse)
+        torch.cuda.synchronize()
+        y = yd.cpu().numpy().reshape(b, n)
+        ref = fp64_dct(x.reshape(b, n).astype(np.float64), inverse)
+        e = max(oracle.rel_rms(y[i], ref[i]) for i in range(b))
+        assert e <= (1e-6 if dt == np.float32 else 1e-14), (inverse, e)
 
 
 def test_dct_batch_65536_x_1024_sample():
