@@ -1280,8 +1280,8 @@ bool dct2_any(bool inverse, int L, const T* x, T* y, const cx<T>* tw, const cx<T
 //   X[i] = conj((F1' - F2' w_{N-i}) / 2)         for N/2 < i < N, primes for the pair (N - i, i)
 // with F1 = Z[i] + conj Z[N-i], F2 = Z[i] - conj Z[N-i], w_k = rtw[k] = exp(-i(2 pi k/n + pi/2));
 // X[0] and X[N] (CCs) or X[0] packed (Perm) from Z[0], X[N/2] = conj Z[N/2].
-// C2R: each slot forms its packed input from X[i] and X[N - i] (two coalesced loads, the second in
-// reverse order), the inverse transform runs, and the reals are stored as plain pairs.
+// C2R: each slot loads X[i], receives X[N - i] through LDS (two rounds, as above) and forms its packed
+// input; the inverse transform runs, and the reals are stored as plain pairs.
 template <typename T, int L>
 __global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_rfft_fwd(const cx<T>* __restrict__ x,
                                                                             cx<T>* __restrict__ out, long long odist,
@@ -1385,7 +1385,7 @@ __global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_rfft_inv(cons
     constexpr int E  = S::E;
     constexpr int TP = S::TPT;
     constexpr int W  = G::W;
-    static_assert(!G::staged, "fused real transform needs whole-thread transforms");
+    static_assert(!G::staged && E % 2 == 0 && G::H <= 2, "fused real transform needs whole-thread transforms");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
     const mfma16 K{};
@@ -1394,46 +1394,61 @@ __global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_rfft_inv(cons
     const int t       = tid % TP;
     const long long g = (long long)blockIdx.x * W + tb;
     const bool valid  = g < count;
-    const cx<T>* X    = spec + g * idist;
-    cx<T> v[E], u[E];
-    // X[i] and X[N - i] of every slot (X[N] is the Nyquist value in CCs; i = 0 reads X[0] twice)
-#pragma unroll
-    for (int k = 0; k < E; ++k)
+    const cx<T>* X   = spec + g * idist;
+    constexpr int HN  = N / 2;
+    auto at           = [&](int i) -> cx<T>& { return lds[laddr<role_direct, W, G::LD>(tb, i)]; };
+    // slot i packs X[i] with X[N - i]; the partner crosses through LDS in two rounds of N/2 values
+    // (X[N], the Nyquist value in CCs, or X[0] in Perm is the partner of slot 0)
+    auto unpack = [&](int i, cx<T> xk, cx<T> xn) -> cx<T>
     {
-        const int i = t + k * TP;
-        v[k]        = { T(0), T(0) };
-        u[k]        = { T(0), T(0) };
-        if (valid)
-        {
-            v[k] = X[i];
-            u[k] = X[i == 0 ? (fmt == 1 ? N : 0) : N - i];
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < E; ++k)
-    {
-        const int i    = t + k * TP;
-        const cx<T> xk = v[k], xn = u[k];
         if (i == 0)
-            v[k] = fmt == 1 ? cx<T>{ xk.re + xn.re, xk.re - xn.re } : cx<T>{ xk.re + xk.im, xk.re - xk.im };
-        else if (2 * i == N)
-            v[k] = { T(2) * xk.re, T(-2) * xk.im };
-        else if (2 * i < N)
+            return fmt == 1 ? cx<T>{ xk.re + xn.re, xk.re - xn.re } : cx<T>{ xk.re + xk.im, xk.re - xk.im };
+        if (2 * i == N)
+            return { T(2) * xk.re, T(-2) * xk.im };
+        if (2 * i < N)
         {
             const cx<T> f1 = { xk.re + xn.re, xk.im - xn.im };
             const cx<T> f2 = { xk.re - xn.re, xk.im + xn.im };
             const cx<T> tt = cmulc(f2, rtw[i]);
-            v[k]           = { f1.re + tt.re, f1.im + tt.im };
+            return { f1.re + tt.re, f1.im + tt.im };
         }
-        else
-        {
-            // pair (k', N - k') with k' = N - i: X[k'] = xn, X[N - k'] = xk
-            const cx<T> f1 = { xn.re + xk.re, xn.im - xk.im };
-            const cx<T> f2 = { xn.re - xk.re, xn.im + xk.im };
-            const cx<T> tt = cmulc(f2, rtw[N - i]);
-            v[k]           = { f1.re - tt.re, -(f1.im - tt.im) };
-        }
+        // pair (k', N - k') with k' = N - i: X[k'] = xn, X[N - k'] = xk
+        const cx<T> f1 = { xn.re + xk.re, xn.im - xk.im };
+        const cx<T> f2 = { xn.re - xk.re, xn.im + xk.im };
+        const cx<T> tt = cmulc(f2, rtw[N - i]);
+        return { f1.re - tt.re, -(f1.im - tt.im) };
+    };
+    cx<T> v[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k)
+        v[k] = valid ? X[t + k * TP] : cx<T>{ T(0), T(0) };
+    const cx<T> x0p = valid ? X[fmt == 1 ? N : 0] : cx<T>{ T(0), T(0) };
+    cx<T> zu[E / 2];
+#pragma unroll
+    for (int k = 0; k < E / 2; ++k)
+        at(t + k * TP) = v[k];
+    lds_barrier();
+#pragma unroll
+    for (int k = E / 2; k < E; ++k)
+    {
+        const int i   = t + k * TP;
+        zu[k - E / 2] = unpack(i, v[k], i == HN ? v[k] : at(N - i));
     }
+    lds_barrier();
+#pragma unroll
+    for (int k = E / 2; k < E; ++k)
+        at(t + k * TP - HN) = v[k];
+    lds_barrier();
+#pragma unroll
+    for (int k = 0; k < E / 2; ++k)
+    {
+        const int i = t + k * TP;
+        v[k]        = unpack(i, v[k], i == 0 ? x0p : at(N - i - HN));
+    }
+#pragma unroll
+    for (int k = E / 2; k < E; ++k)
+        v[k] = zu[k - E / 2];
+    lds_barrier();
     run_pass<T, L, true, role_direct, prec_full, 0>(v, lds, tb, t, tw, K);
     if (valid)
     {
