@@ -37,6 +37,32 @@ def test_real_dft(n, fmt, dt):
     assert e2 <= bound * 2, e2
 
 
+@pytest.mark.parametrize("n,dt", [(512, np.float32), (1 << 16, np.float32), (256, np.float64), (1 << 15, np.float64)])
+@pytest.mark.parametrize("fmt", [0, 1])
+def test_real_one_kernel_ragged_batch(n, dt, fmt):
+    """R2C and C2R in one kernel (power-of-two halves 2^8..2^15 f32, 2^7..2^14 f64) over a batch that
+    leaves the last workgroup part-filled; every transform against FP64"""
+    torch = pytest.importorskip("torch")
+    b = 37
+    cdt = np.complex64 if dt == np.float32 else np.complex128
+    x = oracle.gaussian(88 + n + fmt, n * b, dt).reshape(b, n)
+    p = kfr_amd.dft_plan_real(n, fmt, dt)
+    cs = p.complex_size()
+    xd = torch.from_numpy(x).cuda()
+    Xd = torch.full((b, cs * 2), float("nan"), dtype=xd.dtype, device="cuda")
+    p.execute_batch(Xd, xd, b)
+    torch.cuda.synchronize()
+    X = Xd.cpu().numpy().view(cdt).reshape(b, cs)
+    ref = fp64_rdft(x.astype(np.float64), fmt)
+    bound = 1e-6 if dt == np.float32 else 1e-14
+    assert max(oracle.rel_rms(X[i], ref[i]) for i in range(b)) <= bound
+    yd = torch.full_like(xd, float("nan"))
+    p.execute_batch(yd, torch.from_numpy(ref.astype(cdt).view(dt)).cuda(), b, True)
+    torch.cuda.synchronize()
+    y = yd.cpu().numpy().reshape(b, n)
+    assert max(oracle.rel_rms(y[i] / n, x[i]) for i in range(b)) <= bound
+
+
 @pytest.mark.parametrize("tn", ["f32", "f64"])
 def test_real_dft_matches_reference_outputs(tn):
     dt = np.float32 if tn == "f32" else np.float64
