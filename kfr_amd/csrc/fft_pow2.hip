@@ -1609,12 +1609,15 @@ struct xcorr_args
 };
 
 template <typename T, int L>
-__global__ void __launch_bounds__((shape<T, L>::TPT)) k_conv_xcorr(xcorr_args<T> a)
+__global__ void __launch_bounds__((geom<T, L, role_block>::BS)) k_conv_xcorr(xcorr_args<T> a)
 {
-    using S          = typename geom<T, L, role_conv>::S;
+    using G          = geom<T, L, role_block>;
+    using S          = typename G::S;
     constexpr int N  = S::N;
     constexpr int E  = S::E;
     constexpr int TP = S::TPT;
+    constexpr int HN = N / 2;
+    static_assert(E % 2 == 0 && G::H <= 2, "mirror exchange in halves");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
     const mfma16 K{};
@@ -1627,6 +1630,7 @@ __global__ void __launch_bounds__((shape<T, L>::TPT)) k_conv_xcorr(xcorr_args<T>
     const T* x           = a.x + item * a.dx;
     const T* g           = a.g + item * a.dg;
     const long long base = seg * a.P;
+    auto at              = [&](int i) -> cx<T>& { return lds[laddr<role_block, 1, G::LD>(0, i)]; };
     cx<T> v[E];
 #pragma unroll
     for (int k = 0; k < E; ++k)
@@ -1636,27 +1640,44 @@ __global__ void __launch_bounds__((shape<T, L>::TPT)) k_conv_xcorr(xcorr_args<T>
         v[k].re           = (n < a.P && p < a.nx) ? x[p] : T(0);
         v[k].im           = p < a.ng ? g[p] : T(0);
     }
-    run_pass<T, L, false, role_conv, prec_full, 0>(v, lds, 0, t, a.tw, K);
-    // v[k] = Z[t + k*TPT]; fetch Z[-n] through LDS
+    run_pass<T, L, false, role_block, prec_full, 0>(v, lds, 0, t, a.tw, K);
+    // v[k] = Z[i], i = t + k*TPT. R[i] = G[i] conj(X[i]) / N needs Z[N - i]. X and G are spectra of
+    // real signals, so R[N - i] = conj(R[i]): the upper slots (i >= N/2) receive their partner in a
+    // first LDS round of N/2 values and form R, and the lower slots take conj(R) back in a second
+    // round (slot 0 pairs with itself, as does N/2).
+    auto prod = [&](cx<T> z, cx<T> zm) -> cx<T>
+    {
+        const cx<T> X   = { T(0.5) * (z.re + zm.re), T(0.5) * (z.im - zm.im) };
+        const cx<T> dgi = { z.re - zm.re, z.im + zm.im }; // G = (z - conj zm) / (2i)
+        const cx<T> Gs  = { T(0.5) * dgi.im, -T(0.5) * dgi.re };
+        return { (Gs.re * X.re + Gs.im * X.im) * a.scale, (Gs.im * X.re - Gs.re * X.im) * a.scale };
+    };
 #pragma unroll
-    for (int k = 0; k < E; ++k)
-        lds[laddr<role_conv, 1, N>(0, t + k * TP)] = v[k];
+    for (int k = 0; k < E / 2; ++k)
+        at(t + k * TP) = v[k];
     lds_barrier();
 #pragma unroll
-    for (int k = 0; k < E; ++k)
+    for (int k = E / 2; k < E; ++k)
     {
-        const int n      = t + k * TP;
-        const cx<T> zm   = lds[laddr<role_conv, 1, N>(0, (N - n) & (N - 1))];
-        const cx<T> z    = v[k];
-        const cx<T> X    = { T(0.5) * (z.re + zm.re), T(0.5) * (z.im - zm.im) };
-        // G = (z - conj zm) / (2i)
-        const cx<T> dgi  = { z.re - zm.re, z.im + zm.im };
-        const cx<T> G    = { T(0.5) * dgi.im, -T(0.5) * dgi.re };
-        // G conj(X) / N
-        v[k] = { (G.re * X.re + G.im * X.im) * a.scale, (G.im * X.re - G.re * X.im) * a.scale };
+        const int i = t + k * TP;
+        v[k]        = prod(v[k], i == HN ? v[k] : at(N - i));
     }
     lds_barrier();
-    run_pass<T, L, true, role_conv, prec_full, 0>(v, lds, 0, t, a.tw, K);
+#pragma unroll
+    for (int k = E / 2; k < E; ++k)
+        at(t + k * TP - HN) = v[k];
+    lds_barrier();
+#pragma unroll
+    for (int k = 0; k < E / 2; ++k)
+    {
+        const int i = t + k * TP;
+        if (i == 0)
+            v[k] = prod(v[k], v[k]);
+        else
+            v[k] = cconj(at(N - i - HN));
+    }
+    lds_barrier();
+    run_pass<T, L, true, role_block, prec_full, 0>(v, lds, 0, t, a.tw, K);
     T* out = a.part + w * a.nh;
 #pragma unroll
     for (int k = 0; k < E; ++k)
@@ -1714,8 +1735,8 @@ void conv_os_launch(const conv_os_args<T>& a, hipStream_t s)
 template <typename T, int L>
 void conv_xcorr_launch(const xcorr_args<T>& a, hipStream_t s)
 {
-    using S                  = shape<T, L>;
-    constexpr size_t bytes   = (size_t)(S::N + S::N / 16 + 1) * sizeof(cx<T>);
+    using G                  = geom<T, L, role_block>;
+    constexpr size_t bytes   = G::lds_bytes;
     static const bool lds_ok = (allow_lds(k_conv_xcorr<T, L>, bytes), true);
     (void)lds_ok;
     if (a.count > 0x7fffffffll)
@@ -1724,7 +1745,7 @@ void conv_xcorr_launch(const xcorr_args<T>& a, hipStream_t s)
                     (long long)sizeof(T) * (a.count / a.nseg) * (a.nx + a.ng) +
                         (long long)sizeof(T) * a.count * a.nh,
                     s);
-    hipLaunchKernelGGL((k_conv_xcorr<T, L>), dim3((unsigned)a.count), dim3(S::TPT), bytes, s, a);
+    hipLaunchKernelGGL((k_conv_xcorr<T, L>), dim3((unsigned)a.count), dim3(G::BS), bytes, s, a);
     hip_check(hipGetLastError(), "correlation launch");
 }
 
