@@ -44,11 +44,14 @@ struct len_desc
 
 struct work_item
 {
-    int len;     // index into the length descriptors
-    int inverse; // direction
-    int first;   // first entry in the member list
-    int count;   // transforms in this item (at most the length's transforms per item)
+    int len;   // index into the length descriptors
+    int first; // first entry in the member list
+    int count; // transforms in this item (at most the length's transforms per item)
 };
+
+// A member entry is (transform index << 1) | inverse: the transforms of one length share items
+// whatever their direction (fewer part-filled items than grouping by length and direction).
+constexpr size_t max_transforms = (size_t)1 << 30;
 
 __global__ void k_queue_reset(unsigned* counter)
 {
@@ -66,16 +69,21 @@ __global__ void __launch_bounds__(block) __attribute__((amdgpu_waves_per_eu(4)))
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+    // the next item is claimed when the current one starts, and its index published when the current
+    // one ends, so the atomic's round trip overlaps the transform instead of preceding it
     __shared__ int s_item;
+    if (threadIdx.x == 0)
+        s_item = (int)atomicAdd(counter, 1u);
+    __syncthreads();
     for (;;)
     {
-        if (threadIdx.x == 0)
-            s_item = (int)atomicAdd(counter, 1u);
-        __syncthreads();
         const int it = s_item;
-        __syncthreads();
+        __syncthreads(); // every thread has read s_item before thread 0 overwrites it below
         if (it >= nitems)
             return; // uniform across the workgroup
+        unsigned next = 0;
+        if (threadIdx.x == 0)
+            next = atomicAdd(counter, 1u);
         const work_item I = items[it];
         const len_desc& D = lens[I.len];
         const lds_geom& G = D.G;
@@ -91,8 +99,9 @@ __global__ void __launch_bounds__(block) __attribute__((amdgpu_waves_per_eu(4)))
         c.t     = active ? tid - tb * G.tpt : G.n; // idle threads skip every element loop
         c.valid = active;
         c.g     = 0;
-        const long long off = active ? offs[members[I.first + tb]] : 0;
-        const bool inv      = I.inverse != 0;
+        const int mbr       = active ? members[I.first + tb] : 0;
+        const long long off = active ? offs[mbr >> 1] : 0;
+        const bool inv      = (mbr & 1) != 0;
         const int n         = D.n;
         const cx<T>* ch     = tab + D.chirp;
 
@@ -161,7 +170,9 @@ __global__ void __launch_bounds__(block) __attribute__((amdgpu_waves_per_eu(4)))
                 v = cmul(cconj(v), ch[i]);
             out[off + i] = inv ? cconj(v) : v;
         }
-        __syncthreads(); // LDS is rewritten by the next item
+        if (threadIdx.x == 0)
+            s_item = (int)next;
+        __syncthreads(); // LDS is rewritten by the next item, whose index is now in s_item
     }
 }
 
@@ -355,10 +366,10 @@ void autotune(host_len<T>& h, int force_order, int force_ps, hipStream_t s)
         std::vector<int> mem(nt);
         std::vector<long long> of(nt);
         for (int i = 0; i < nitems; ++i)
-            it[i] = { 0, 0, i * tpb, tpb };
+            it[i] = { 0, i * tpb, tpb };
         for (long long j = 0; j < nt; ++j)
         {
-            mem[j] = (int)j;
+            mem[j] = (int)(j << 1);
             of[j]  = j * (long long)h.n;
         }
         host_len<T> hh = h;
@@ -466,7 +477,7 @@ gpu_batch<T>::gpu_batch(size_t count, const size_t* sizes, const int* inverse, b
     using namespace bt;
     if (count == 0)
         throw std::invalid_argument("empty batch");
-    if (count > 0x7fffffff)
+    if (count >= max_transforms)
         throw std::invalid_argument("batch has too many transforms");
     p->count          = count;
     hipStream_t s     = library_stream();
@@ -560,21 +571,22 @@ gpu_batch<T>::gpu_batch(size_t count, const size_t* sizes, const int* inverse, b
             bluestein_tables<T>(h.n, h.m, tb + h.chirp, tb + h.H, s);
     }
     hip_check(hipStreamSynchronize(s), "batch tables");
-    // group transforms by (length, direction) and cut the groups into items, costliest first
-    std::map<std::pair<int, int>, std::vector<int>> groups;
+    // group transforms by length (each member carries its own direction) and cut the groups into
+    // items, costliest first
+    std::map<int, std::vector<int>> groups;
     for (size_t i = 0; i < count; ++i)
-        groups[{ index[sizes[i]], inverse && inverse[i] ? 1 : 0 }].push_back((int)i);
+        groups[index[sizes[i]]].push_back((int)(i << 1) | (inverse && inverse[i] ? 1 : 0));
     std::vector<int> mem;
     mem.reserve(count);
     std::vector<std::pair<double, work_item>> its;
     for (auto& [key, ids] : groups)
     {
-        const host_len<T>& h = L[key.first];
+        const host_len<T>& h = L[key];
         const double unit    = (double)h.m * std::log2((double)std::max<size_t>(2, h.m)) * (h.bs ? 2.0 : 1.0);
         for (size_t a = 0; a < ids.size(); a += h.tpb)
         {
             const int cnt = (int)std::min<size_t>(h.tpb, ids.size() - a);
-            work_item w{ key.first, key.second, (int)mem.size(), cnt };
+            work_item w{ key, (int)mem.size(), cnt };
             for (int j = 0; j < cnt; ++j)
                 mem.push_back(ids[a + j]);
             its.push_back({ unit * cnt, w });
