@@ -410,8 +410,11 @@ __device__ __forceinline__ void lds_barrier()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// pass P of the schedule on the registers v (strided set in, strided set out after the exchange)
-template <typename T, int L, bool INV, int ROLE, int PREC, int P>
+// pass P of the schedule on the registers v (strided set in, strided set out after the exchange).
+// MIRROR0 (first pass of two butterflies per thread): the second butterfly of thread t is
+// 2 TPT-1-t instead of t + TPT, so that x[q] and x[N-1-q] are inputs of the same thread's two
+// butterflies (the mirrored input of the one-kernel DCT-II then needs no exchange).
+template <typename T, int L, bool INV, int ROLE, int PREC, int P, bool MIRROR0 = false>
 __device__ __forceinline__ void run_pass(regs<T, L, ROLE, PREC>& v, cx<T>* lds, int tb, int t, const cx<T>* tw,
                                          const mfma16& K)
 {
@@ -484,7 +487,7 @@ __device__ __forceinline__ void run_pass(regs<T, L, ROLE, PREC>& v, cx<T>* lds, 
 #pragma unroll
             for (int m = h * MH; m < (h + 1) * MH; ++m)
             {
-                const int j    = t + m * TP;
+                const int j    = MIRROR0 && m >= M / 2 ? TP - 1 - t + m * TP : t + m * TP;
                 const int kt   = j & (NS - 1);
                 const int base = (j - kt) * R + kt - h * G::NH;
 #pragma unroll
@@ -511,6 +514,7 @@ __device__ __forceinline__ void run_pass(regs<T, L, ROLE, PREC>& v, cx<T>* lds, 
         }
         run_pass<T, L, INV, ROLE, PREC, P + 1>(v, lds, tb, t, tw, K);
     }
+    static_assert(!MIRROR0 || (P == 0 && NS == 1 && M == 2 && P + 1 < S::NP), "mirrored first pass: two butterflies");
 }
 
 // power-of-two scale 2^k of a value array whose largest |re|, |im| is mx: the scaled values stay below
@@ -1043,21 +1047,50 @@ __global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_dct2(const T*
     const long long n = 2ll * N;
     auto at           = [&](int i) -> cx<T>& { return lds[laddr<role_direct, W, G::LD>(tb, i)]; };
     cx<T> v[E];
-#pragma unroll
-    for (int k = 0; k < E / 2; ++k)
+    constexpr int M0 = E >> S::lr(0); // butterflies per thread in the first pass
+    if constexpr (M0 == 2 && S::NP > 1)
     {
-        quad<T> q4 = { T(0), T(0), T(0), T(0) };
-        if (valid)
-            q4 = reinterpret_cast<const quad<T>*>(x + g * n)[t + k * TP];
-        v[k]                                 = { q4.a, q4.c };
-        at((E / 2 - 1 - k) * TP + (TP - 1 - t)) = { q4.d, q4.b }; // z[N-1-q] - N/2
-    }
-    lds_barrier();
+        // The first pass's butterflies of thread t are j0 = t and j1 = 2 TPT-1-t (inputs j + r N/R0,
+        // N/R0 = 2 TPT; slot 2r holds input r of j0, slot 2r+1 input r of j1). For r < R0/2, quad
+        // q = j0 + r N/R0 holds z[q] (input r of j0) and z[N-1-q] (input R0-1-r of j1), and quad
+        // q = j1 + r N/R0 holds z[q] (input r of j1) and z[N-1-q] (input R0-1-r of j0): the thread
+        // loads all of its inputs itself, no exchange
+        constexpr int R0 = E / 2;
+        const quad<T>* xq = reinterpret_cast<const quad<T>*>(x + g * n);
 #pragma unroll
-    for (int k = E / 2; k < E; ++k)
-        v[k] = at((k - E / 2) * TP + t);
-    lds_barrier();
-    run_pass<T, L, false, role_direct, prec_full, 0>(v, lds, tb, t, tw, K);
+        for (int r = 0; r < R0 / 2; ++r)
+        {
+            quad<T> qa = { T(0), T(0), T(0), T(0) }, qb = qa;
+            if (valid)
+            {
+                qa = xq[t + r * (2 * TP)];
+                qb = xq[2 * TP - 1 - t + r * (2 * TP)];
+            }
+            v[2 * r]                = { qa.a, qa.c };
+            v[1 + 2 * (R0 - 1 - r)] = { qa.d, qa.b };
+            v[1 + 2 * r]            = { qb.a, qb.c };
+            v[2 * (R0 - 1 - r)]     = { qb.d, qb.b };
+        }
+        run_pass<T, L, false, role_direct, prec_full, 0, true>(v, lds, tb, t, tw, K);
+    }
+    else
+    {
+#pragma unroll
+        for (int k = 0; k < E / 2; ++k)
+        {
+            quad<T> q4 = { T(0), T(0), T(0), T(0) };
+            if (valid)
+                q4 = reinterpret_cast<const quad<T>*>(x + g * n)[t + k * TP];
+            v[k]                                    = { q4.a, q4.c };
+            at((E / 2 - 1 - k) * TP + (TP - 1 - t)) = { q4.d, q4.b }; // z[N-1-q] - N/2
+        }
+        lds_barrier();
+#pragma unroll
+        for (int k = E / 2; k < E; ++k)
+            v[k] = at((k - E / 2) * TP + t);
+        lds_barrier();
+        run_pass<T, L, false, role_direct, prec_full, 0>(v, lds, tb, t, tw, K);
+    }
     // v[k] = Z[t + k TPT]. Output pair i, n - i needs Z[i] and Z[N - i] (Z[0] for i = 0): the slots
     // k >= E/2 (i >= N/2) find their partners among the slots k < E/2 of the workgroup's transform
     // after round one, and those after round two, each slot emitting its outputs at once.
