@@ -888,12 +888,15 @@ __global__ void __launch_bounds__((geom<T, L, ROLE, PREC>::BS)) k_pow2(args<T> a
             lds_barrier();
             if (g2 < a.count)
             {
+                // one running pointer: the outputs of a thread are TPT << lo apart
+                const long long step = (long long)TP << a.lo;
+                long long o          = base + ((long long)(i0 + h * (E / H) * TP) << a.lo);
 #pragma unroll
                 for (int k = h * (E / H); k < (h + 1) * (E / H); ++k)
                 {
                     const int i = i0 + k * TP;
-                    st_elem<T, PREC>(a.out, base + ((long long)i << a.lo),
-                                     lds[laddr<ROLE, W, G::LD>(tb2, i - h * G::NH)]);
+                    st_elem<T, PREC>(a.out, o, lds[laddr<ROLE, W, G::LD>(tb2, i - h * G::NH)]);
+                    o += step;
                 }
             }
         }
