@@ -1018,7 +1018,9 @@ inline int log2_exact(size_t n)
 // the even and odd samples (from Z[k] and Z[N-k]); y[k] = Re(V[k] W_4n^k), y[n-k] = -Im(V[k] W_4n^k),
 // y[N] = Re((Re Z0 - Im Z0) W_4n^N). Thread t holds z[t + k TPT]: slots k < E/2 take z[q] straight
 // from one coalesced load of x[4q..4q+3], whose other half z[N-1-q] is slot E-1-k of thread TPT-1-t and
-// crosses through LDS; after the passes the partners Z[N-i] cross through LDS in two rounds of N/2.
+// crosses through LDS (no exchange when the first pass has two butterflies per thread: they are
+// assigned in mirror pairs); after the passes one LDS round of N/2 values brings Z[N-i] to the slot i
+// < N/2, which writes the outputs of both i and N-i.
 template <typename T>
 struct alignas(4 * sizeof(T)) quad
 {
@@ -1094,9 +1096,7 @@ __global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_dct2(const T*
         lds_barrier();
         run_pass<T, L, false, role_direct, prec_full, 0>(v, lds, tb, t, tw, K);
     }
-    // v[k] = Z[t + k TPT]. Output pair i, n - i needs Z[i] and Z[N - i] (Z[0] for i = 0): the slots
-    // k >= E/2 (i >= N/2) find their partners among the slots k < E/2 of the workgroup's transform
-    // after round one, and those after round two, each slot emitting its outputs at once.
+    // v[k] = Z[t + k TPT]. Output pair i, n - i needs Z[i] and Z[N - i] (Z[0] for i = 0).
     T* yb     = y + g * n;
     auto emit = [&](int i, cx<T> a, cx<T> c)
     {
@@ -1119,17 +1119,8 @@ __global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_dct2(const T*
         yb[i]          = V.re * w.re - V.im * w.im;
         yb[n - i]      = -(V.re * w.im + V.im * w.re);
     };
-#pragma unroll
-    for (int k = 0; k < E / 2; ++k)
-        at(t + k * TP) = v[k];
-    lds_barrier();
-#pragma unroll
-    for (int k = E / 2; k < E; ++k)
-    {
-        const int i = t + k * TP;
-        emit(i, v[k], i == HN ? v[k] : at(N - i));
-    }
-    lds_barrier();
+    // the upper slots publish Z; each lower slot i reads Z[N-i] and writes the outputs of both i and
+    // N - i; slot N/2 pairs with itself
 #pragma unroll
     for (int k = E / 2; k < E; ++k)
         at(t + k * TP - HN) = v[k];
@@ -1138,8 +1129,19 @@ __global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_dct2(const T*
     for (int k = 0; k < E / 2; ++k)
     {
         const int i = t + k * TP;
-        emit(i, v[k], i == 0 ? v[k] : at(N - i - HN));
+        if (i == 0)
+            emit(0, v[k], v[k]);
+        else
+        {
+            const cx<T> c = at(N - i - HN);
+            emit(i, v[k], c);
+            emit(N - i, c, v[k]);
+        }
     }
+#pragma unroll
+    for (int k = E / 2; k < E; ++k)
+        if (t + k * TP == HN)
+            emit(HN, v[k], v[k]);
 }
 
 // DCT-III (dct_plan's inverse, unscaled) in one kernel, the mirror image of k_dct2: with V[k] = y'[k] W_4n^k
@@ -1182,9 +1184,8 @@ __global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_dct3(const T*
         const cx<T> w1 = { -w.im, -w.re };
         return { T(0.5) * (a * w.re + b * w1.re), T(0.5) * (b * w1.im - a * w.im) };
     };
-    // slot i holds H[i] (two coalesced loads, one in reverse order); the partner H[N - i] crosses
-    // through LDS in two rounds of N/2 values as in k_dct2, H[N] (the partner of slot 0) is computed
-    // by every thread from y[N]
+    // slot i holds H[i] (two coalesced loads, one in reverse order); H[N] (the partner of slot 0) is
+    // computed by every thread from y[N]
     auto pack = [&](int i, cx<T> a, cx<T> c) -> cx<T>
     {
         // (a + conj c) + i (a - conj c) conj(W_n^i)
@@ -1209,18 +1210,8 @@ __global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_dct3(const T*
     }
     const T yN      = valid ? yb[N] : T(0);
     const cx<T> hN  = Hs(false, yN, yN, w4n[N]);
-    cx<T> zu[E / 2];
-#pragma unroll
-    for (int k = 0; k < E / 2; ++k)
-        at(t + k * TP) = v[k];
-    lds_barrier();
-#pragma unroll
-    for (int k = E / 2; k < E; ++k)
-    {
-        const int i   = t + k * TP;
-        zu[k - E / 2] = pack(i, v[k], i == HN ? v[k] : at(N - i));
-    }
-    lds_barrier();
+    // the upper slots publish H; each lower slot i reads H[N-i], packs both slot i and slot N-i, and
+    // leaves the latter in the position it read; the upper slots collect theirs after one barrier
 #pragma unroll
     for (int k = E / 2; k < E; ++k)
         at(t + k * TP - HN) = v[k];
@@ -1229,11 +1220,23 @@ __global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_dct3(const T*
     for (int k = 0; k < E / 2; ++k)
     {
         const int i = t + k * TP;
-        v[k]        = pack(i, v[k], i == 0 ? hN : at(N - i - HN));
+        if (i == 0)
+            v[k] = pack(0, v[k], hN);
+        else
+        {
+            cx<T>& slot    = at(N - i - HN);
+            const cx<T> hp = slot;
+            slot           = pack(N - i, hp, v[k]);
+            v[k]           = pack(i, v[k], hp);
+        }
     }
+    lds_barrier();
 #pragma unroll
     for (int k = E / 2; k < E; ++k)
-        v[k] = zu[k - E / 2];
+    {
+        const int i = t + k * TP;
+        v[k]        = i == HN ? pack(HN, v[k], v[k]) : at(i - HN);
+    }
     lds_barrier();
     run_pass<T, L, true, role_direct, prec_full, 0>(v, lds, tb, t, tw, K);
     // v[k] = u[t + k TPT]; the upper half of the slots crosses to the lower half in mirror order
@@ -1310,14 +1313,15 @@ bool dct2_any(bool inverse, int L, const T* x, T* y, const cx<T>* tw, const cx<T
 
 // ---- Real transforms in one kernel (dft_plan_real, to_fmt / from_fmt of src/dft/fft-impl.hpp:1799-1896)
 // for half lengths N = 2^L that fit one workgroup. The n = 2N reals are N complex values z.
-// R2C: Z = DFT_N(z) in registers (slot k of thread t holds Z[t + k TPT]); the partner Z[N - i] of
-// every slot crosses through LDS in two rounds of N/2 values, and each slot writes its own output
+// R2C: Z = DFT_N(z) in registers (slot k of thread t holds Z[t + k TPT]); one LDS round of N/2 values
+// brings Z[N - i] to every slot i < N/2, which writes the outputs of both i and N - i
 //   X[i] = (F1 + F2 w_i) / 2                     for 0 < i < N/2
 //   X[i] = conj((F1' - F2' w_{N-i}) / 2)         for N/2 < i < N, primes for the pair (N - i, i)
 // with F1 = Z[i] + conj Z[N-i], F2 = Z[i] - conj Z[N-i], w_k = rtw[k] = exp(-i(2 pi k/n + pi/2));
 // X[0] and X[N] (CCs) or X[0] packed (Perm) from Z[0], X[N/2] = conj Z[N/2].
-// C2R: each slot loads X[i], receives X[N - i] through LDS (two rounds, as above) and forms its packed
-// input; the inverse transform runs, and the reals are stored as plain pairs.
+// C2R: each slot loads X[i]; the lower half of the slots receives X[N - i] through LDS, forms the
+// packed inputs of both slots i and N - i, and returns the latter the same way; the inverse transform
+// runs, and the reals are stored as plain pairs.
 template <typename T, int L>
 __global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_rfft_fwd(const cx<T>* __restrict__ x,
                                                                             cx<T>* __restrict__ out, long long odist,
@@ -1385,17 +1389,8 @@ __global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_rfft_fwd(cons
             o[i]           = { T(0.5) * (f1.re - tt.re), -T(0.5) * (f1.im - tt.im) };
         }
     };
-#pragma unroll
-    for (int k = 0; k < E / 2; ++k)
-        at(t + k * TP) = v[k];
-    lds_barrier();
-#pragma unroll
-    for (int k = E / 2; k < E; ++k)
-    {
-        const int i = t + k * TP;
-        emit(i, v[k], i == HN ? v[k] : at(N - i));
-    }
-    lds_barrier();
+    // the upper slots publish Z; each lower slot i reads Z[N-i] and writes the outputs of both i and
+    // N - i (the second run in reverse order, still coalesced); slot N/2 pairs with itself
 #pragma unroll
     for (int k = E / 2; k < E; ++k)
         at(t + k * TP - HN) = v[k];
@@ -1404,8 +1399,19 @@ __global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_rfft_fwd(cons
     for (int k = 0; k < E / 2; ++k)
     {
         const int i = t + k * TP;
-        emit(i, v[k], i == 0 ? v[k] : at(N - i - HN));
+        if (i == 0)
+            emit(0, v[k], v[k]);
+        else
+        {
+            const cx<T> c = at(N - i - HN);
+            emit(i, v[k], c);
+            emit(N - i, c, v[k]);
+        }
     }
+#pragma unroll
+    for (int k = E / 2; k < E; ++k)
+        if (t + k * TP == HN)
+            emit(HN, v[k], v[k]);
 }
 
 template <typename T, int L>
@@ -1433,8 +1439,7 @@ __global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_rfft_inv(cons
     const cx<T>* X   = spec + g * idist;
     constexpr int HN  = N / 2;
     auto at           = [&](int i) -> cx<T>& { return lds[laddr<role_direct, W, G::LD>(tb, i)]; };
-    // slot i packs X[i] with X[N - i]; the partner crosses through LDS in two rounds of N/2 values
-    // (X[N], the Nyquist value in CCs, or X[0] in Perm is the partner of slot 0)
+    // the packed input of slot i from X[i] and X[N - i] (from_fmt)
     auto unpack = [&](int i, cx<T> xk, cx<T> xn) -> cx<T>
     {
         if (i == 0)
@@ -1459,18 +1464,10 @@ __global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_rfft_inv(cons
     for (int k = 0; k < E; ++k)
         v[k] = valid ? X[t + k * TP] : cx<T>{ T(0), T(0) };
     const cx<T> x0p = valid ? X[fmt == 1 ? N : 0] : cx<T>{ T(0), T(0) };
-    cx<T> zu[E / 2];
-#pragma unroll
-    for (int k = 0; k < E / 2; ++k)
-        at(t + k * TP) = v[k];
-    lds_barrier();
-#pragma unroll
-    for (int k = E / 2; k < E; ++k)
-    {
-        const int i   = t + k * TP;
-        zu[k - E / 2] = unpack(i, v[k], i == HN ? v[k] : at(N - i));
-    }
-    lds_barrier();
+    // the upper slots publish X; each lower slot i reads X[N-i], forms both its own value and that of
+    // slot N-i, and leaves the latter in the position it read (no other thread reads or writes it);
+    // the upper slots collect theirs after one barrier. Slot 0 pairs with X[N] (CCs) or X[0] (Perm),
+    // slot N/2 with itself.
 #pragma unroll
     for (int k = E / 2; k < E; ++k)
         at(t + k * TP - HN) = v[k];
@@ -1479,11 +1476,23 @@ __global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_rfft_inv(cons
     for (int k = 0; k < E / 2; ++k)
     {
         const int i = t + k * TP;
-        v[k]        = unpack(i, v[k], i == 0 ? x0p : at(N - i - HN));
+        if (i == 0)
+            v[k] = unpack(0, v[k], x0p);
+        else
+        {
+            cx<T>& slot    = at(N - i - HN);
+            const cx<T> xp = slot;
+            slot           = unpack(N - i, xp, v[k]);
+            v[k]           = unpack(i, v[k], xp);
+        }
     }
+    lds_barrier();
 #pragma unroll
     for (int k = E / 2; k < E; ++k)
-        v[k] = zu[k - E / 2];
+    {
+        const int i = t + k * TP;
+        v[k]        = i == HN ? unpack(HN, v[k], v[k]) : at(i - HN);
+    }
     lds_barrier();
     run_pass<T, L, true, role_direct, prec_full, 0>(v, lds, tb, t, tw, K);
     if (valid)
@@ -1678,9 +1687,9 @@ __global__ void __launch_bounds__((geom<T, L, role_block>::BS)) k_conv_xcorr(xco
     }
     run_pass<T, L, false, role_block, prec_full, 0>(v, lds, 0, t, a.tw, K);
     // v[k] = Z[i], i = t + k*TPT. R[i] = G[i] conj(X[i]) / N needs Z[N - i]. X and G are spectra of
-    // real signals, so R[N - i] = conj(R[i]): the upper slots (i >= N/2) receive their partner in a
-    // first LDS round of N/2 values and form R, and the lower slots take conj(R) back in a second
-    // round (slot 0 pairs with itself, as does N/2).
+    // real signals, so R[N - i] = conj(R[i]): the upper slots (i >= N/2) publish Z, each lower slot
+    // reads its partner, forms R[i] and leaves conj(R[i]) = R[N - i] in the position it read, and the
+    // upper slots collect theirs (slot 0 pairs with itself, as does N/2).
     auto prod = [&](cx<T> z, cx<T> zm) -> cx<T>
     {
         const cx<T> X   = { T(0.5) * (z.re + zm.re), T(0.5) * (z.im - zm.im) };
@@ -1688,17 +1697,6 @@ __global__ void __launch_bounds__((geom<T, L, role_block>::BS)) k_conv_xcorr(xco
         const cx<T> Gs  = { T(0.5) * dgi.im, -T(0.5) * dgi.re };
         return { (Gs.re * X.re + Gs.im * X.im) * a.scale, (Gs.im * X.re - Gs.re * X.im) * a.scale };
     };
-#pragma unroll
-    for (int k = 0; k < E / 2; ++k)
-        at(t + k * TP) = v[k];
-    lds_barrier();
-#pragma unroll
-    for (int k = E / 2; k < E; ++k)
-    {
-        const int i = t + k * TP;
-        v[k]        = prod(v[k], i == HN ? v[k] : at(N - i));
-    }
-    lds_barrier();
 #pragma unroll
     for (int k = E / 2; k < E; ++k)
         at(t + k * TP - HN) = v[k];
@@ -1710,7 +1708,18 @@ __global__ void __launch_bounds__((geom<T, L, role_block>::BS)) k_conv_xcorr(xco
         if (i == 0)
             v[k] = prod(v[k], v[k]);
         else
-            v[k] = cconj(at(N - i - HN));
+        {
+            cx<T>& slot = at(N - i - HN);
+            v[k]        = prod(v[k], slot);
+            slot        = cconj(v[k]); // R[N - i], for the slot that published Z[N - i]
+        }
+    }
+    lds_barrier();
+#pragma unroll
+    for (int k = E / 2; k < E; ++k)
+    {
+        const int i = t + k * TP;
+        v[k]        = i == HN ? prod(v[k], v[k]) : at(i - HN);
     }
     lds_barrier();
     run_pass<T, L, true, role_block, prec_full, 0>(v, lds, 0, t, a.tw, K);
