@@ -215,6 +215,24 @@ __device__ __forceinline__ void bfly16(cx<T>* v)
         v[i] = x[i];
 }
 
+// An opaque copy of v (an empty asm statement that the compiler must assume rewrites the register).
+// Indices derived from the copy are not merged with those derived from v, so a kernel recomputes its
+// cheap per-slot offsets in each phase instead of keeping dozens of them live (and spilled) across
+// the phases in between.
+__device__ __forceinline__ int fresh(int v)
+{
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+// Makes a value exist in registers at this point: the arithmetic that forms it cannot sink below,
+// so the (more numerous) operands it was formed from die here.
+template <typename T>
+__device__ __forceinline__ void settle(cx<T>& a)
+{
+    asm volatile("" : "+v"(a.re), "+v"(a.im));
+}
+
 // f(std::integral_constant<int, I>) for I = 0 .. N-1, unrolled at compile time
 template <int N, typename F>
 __device__ __forceinline__ void static_for(F&& f)

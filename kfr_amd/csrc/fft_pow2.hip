@@ -155,6 +155,9 @@ struct geom
     static constexpr int LD = ROLE == role_row ? NH + 1 : NH; // LDS distance between transforms
     static constexpr int logical = ROLE == role_row ? W * LD : W * NH;
     static constexpr size_t lds_bytes = (size_t)(logical + logical / 16 + 1) * sizeof(cx<T>);
+    // waves per SIMD that the workgroups' LDS lets one CU hold, up to 4 (128 VGPRs): the one-kernel
+    // real and DCT kernels are compiled for it so that their registers do not cost a workgroup per CU
+    static constexpr int lds_waves = std::min<int>(4, (int)(163840 / lds_bytes) * (BS / 64) / 4);
     // short transforms (under 16 threads each) load and store through LDS in whole-workgroup runs
     static constexpr bool staged = ROLE == role_direct && S::TPT < 16;
     static_assert(BS <= 1024, "workgroup too large");
@@ -1028,10 +1031,9 @@ struct alignas(4 * sizeof(T)) quad
 };
 
 template <typename T, int L>
-__global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_dct2(const T* __restrict__ x, T* __restrict__ y,
-                                                                        const cx<T>* __restrict__ tw,
-                                                                        const cx<T>* __restrict__ wn,
-                                                                        const cx<T>* __restrict__ w4n, long long count)
+__global__ void __launch_bounds__((geom<T, L, role_direct>::BS), (geom<T, L, role_direct>::lds_waves))
+k_dct2(const T* __restrict__ x, T* __restrict__ y, const cx<T>* __restrict__ tw, const cx<T>* __restrict__ wn,
+       const cx<T>* __restrict__ w4n, long long count)
 {
     using G          = geom<T, L, role_direct>;
     using S          = typename G::S;
@@ -1076,7 +1078,7 @@ __global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_dct2(const T*
             v[1 + 2 * r]            = { qb.a, qb.c };
             v[2 * (R0 - 1 - r)]     = { qb.d, qb.b };
         }
-        run_pass<T, L, false, role_direct, prec_full, 0, true>(v, lds, tb, t, tw, K);
+        run_pass<T, L, false, role_direct, prec_full, 0, true>(v, lds, tb, fresh(t), tw, K);
     }
     else
     {
@@ -1094,22 +1096,20 @@ __global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_dct2(const T*
         for (int k = E / 2; k < E; ++k)
             v[k] = at((k - E / 2) * TP + t);
         lds_barrier();
-        run_pass<T, L, false, role_direct, prec_full, 0>(v, lds, tb, t, tw, K);
+        run_pass<T, L, false, role_direct, prec_full, 0>(v, lds, tb, fresh(t), tw, K);
     }
     // v[k] = Z[t + k TPT]. Output pair i, n - i needs Z[i] and Z[N - i] (Z[0] for i = 0).
     T* yb     = y + g * n;
+    // i = 0: V[0] = Re Z0 + Im Z0 and V[N] = Re Z0 - Im Z0, both real
+    auto emit0 = [&](cx<T> a)
+    {
+        yb[0]          = a.re + a.im;
+        const cx<T> wN = w4n[N];
+        yb[N]          = (a.re - a.im) * wN.re;
+    };
+    // 0 < i < N
     auto emit = [&](int i, cx<T> a, cx<T> c)
     {
-        if (!valid)
-            return;
-        if (i == 0)
-        {
-            // V[0] = Re Z0 + Im Z0 and V[N] = Re Z0 - Im Z0, both real
-            yb[0]          = a.re + a.im;
-            const cx<T> wN = w4n[N];
-            yb[N]          = (a.re - a.im) * wN.re;
-            return;
-        }
         // A = (Z[i] + conj Z[N-i]) / 2, B = (Z[i] - conj Z[N-i]) / 2i, V = A + W_n^i B
         const cx<T> A  = { T(0.5) * (a.re + c.re), T(0.5) * (a.im - c.im) };
         const cx<T> B  = { T(0.5) * (a.im + c.im), -T(0.5) * (a.re - c.re) };
@@ -1120,17 +1120,21 @@ __global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_dct2(const T*
         yb[n - i]      = -(V.re * w.im + V.im * w.re);
     };
     // the upper slots publish Z; each lower slot i reads Z[N-i] and writes the outputs of both i and
-    // N - i; slot N/2 pairs with itself
+    // N - i; slot N/2 pairs with itself. The indices come from a fresh copy of t, so none is held
+    // through the passes.
+    const int ts = fresh(t);
 #pragma unroll
     for (int k = E / 2; k < E; ++k)
-        at(t + k * TP - HN) = v[k];
+        at(ts + k * TP - HN) = v[k];
     lds_barrier();
+    if (!valid)
+        return;
 #pragma unroll
     for (int k = 0; k < E / 2; ++k)
     {
-        const int i = t + k * TP;
-        if (i == 0)
-            emit(0, v[k], v[k]);
+        const int i = ts + k * TP;
+        if (k == 0 && ts == 0)
+            emit0(v[k]);
         else
         {
             const cx<T> c = at(N - i - HN);
@@ -1138,10 +1142,8 @@ __global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_dct2(const T*
             emit(N - i, c, v[k]);
         }
     }
-#pragma unroll
-    for (int k = E / 2; k < E; ++k)
-        if (t + k * TP == HN)
-            emit(HN, v[k], v[k]);
+    if (ts == 0)
+        emit(HN, v[E / 2], v[E / 2]);
 }
 
 // DCT-III (dct_plan's inverse, unscaled) in one kernel, the mirror image of k_dct2: with V[k] = y'[k] W_4n^k
@@ -1150,10 +1152,9 @@ __global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_dct2(const T*
 // x[4q..4q+3] = (u[q].re, u[N-1-q].im, u[q].im, u[N-1-q].re) leaves as one 16-byte store per q < N/2,
 // u[N-1-q] crossing through LDS from slot E-1-k of thread TPT-1-t.
 template <typename T, int L>
-__global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_dct3(const T* __restrict__ y, T* __restrict__ x,
-                                                                        const cx<T>* __restrict__ tw,
-                                                                        const cx<T>* __restrict__ wn,
-                                                                        const cx<T>* __restrict__ w4n, long long count)
+__global__ void __launch_bounds__((geom<T, L, role_direct>::BS), (geom<T, L, role_direct>::lds_waves))
+k_dct3(const T* __restrict__ y, T* __restrict__ x, const cx<T>* __restrict__ tw, const cx<T>* __restrict__ wn,
+       const cx<T>* __restrict__ w4n, long long count)
 {
     using G          = geom<T, L, role_direct>;
     using S          = typename G::S;
@@ -1173,7 +1174,9 @@ __global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_dct3(const T*
     const bool valid  = g < count;
     constexpr long long n = 2ll * N;
     auto at               = [&](int i) -> cx<T>& { return lds[laddr<role_direct, W, G::LD>(tb, i)]; };
-    const T* yb           = y + g * n;
+    // a thread past the last transform loads the last one (its results are never stored), so the loads
+    // need no branches
+    const T* yb = y + (valid ? g : count - 1) * n;
     // H[kk] = (conj V[kk] + V[n-kk]) / 2 from a = y[kk], b = y[n-kk] and w = W_4n^kk, using
     // W_4n^(n-kk) = (-w.im, -w.re) (the table values swapped, so exactly the rounded table entry);
     // kk = 0 reads y[0] for both (V[n] = V[0]) and halves it
@@ -1186,13 +1189,13 @@ __global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_dct3(const T*
     };
     // slot i holds H[i] (two coalesced loads, one in reverse order); H[N] (the partner of slot 0) is
     // computed by every thread from y[N]
-    auto pack = [&](int i, cx<T> a, cx<T> c) -> cx<T>
+    // (a + conj c) + i (a - conj c) u with u = conj(W_n^i); slot N - i has u = -W_n^i, since
+    // W_n^(N-i) = -conj(W_n^i), so one table value serves both slots of a pair
+    auto pack = [&](cx<T> a, cx<T> c, cx<T> u) -> cx<T>
     {
-        // (a + conj c) + i (a - conj c) conj(W_n^i)
         const cx<T> sm = { a.re + c.re, a.im - c.im };
         const cx<T> d  = { a.re - c.re, a.im + c.im };
-        const cx<T> w  = wn[i];
-        const cx<T> e  = { d.re * w.re + d.im * w.im, d.im * w.re - d.re * w.im };
+        const cx<T> e  = cmul(d, u);
         return { sm.re - e.im, sm.im + e.re };
     };
     cx<T> v[E];
@@ -1200,56 +1203,58 @@ __global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_dct3(const T*
     for (int k = 0; k < E; ++k)
     {
         const int i = t + k * TP;
-        T ya = T(0), yr = T(0);
-        if (valid)
-        {
-            ya = yb[i];
-            yr = yb[i == 0 ? 0 : n - i];
-        }
-        v[k] = Hs(i == 0, ya, yr, w4n[i]);
+        v[k]        = Hs(i == 0, yb[i], yb[i == 0 ? 0 : n - i], w4n[i]);
+        settle(v[k]);
     }
-    const T yN      = valid ? yb[N] : T(0);
-    const cx<T> hN  = Hs(false, yN, yN, w4n[N]);
+    const T yN     = yb[N];
+    const cx<T> hN = Hs(false, yN, yN, w4n[N]);
     // the upper slots publish H; each lower slot i reads H[N-i], packs both slot i and slot N-i, and
     // leaves the latter in the position it read; the upper slots collect theirs after one barrier
 #pragma unroll
     for (int k = E / 2; k < E; ++k)
         at(t + k * TP - HN) = v[k];
     lds_barrier();
+    // the pack phase recomputes its offsets rather than keep the load phase's; slot 0 (k = 0, t = 0)
+    // pairs with H[N] and slot N/2 (k = E/2, t = 0) with itself, selected without branches: slot 0
+    // reads and rewrites position N/2 of the transform's LDS area, which nothing reads before the
+    // passes overwrite it, and slot N/2 reads position 0
+    const int tp = fresh(t);
 #pragma unroll
     for (int k = 0; k < E / 2; ++k)
     {
-        const int i = t + k * TP;
-        if (i == 0)
-            v[k] = pack(0, v[k], hN);
-        else
-        {
-            cx<T>& slot    = at(N - i - HN);
-            const cx<T> hp = slot;
-            slot           = pack(N - i, hp, v[k]);
-            v[k]           = pack(i, v[k], hp);
-        }
+        const int i     = tp + k * TP;
+        const bool zero = k == 0 && tp == 0;
+        cx<T>& slot     = at(N - i - HN);
+        const cx<T> s0  = slot;
+        const cx<T> hp  = zero ? hN : s0;
+        const cx<T> w   = wn[i];
+        slot            = pack(hp, v[k], { -w.re, -w.im });
+        v[k]            = pack(v[k], hp, cconj(w));
+        settle(v[k]);
     }
     lds_barrier();
 #pragma unroll
     for (int k = E / 2; k < E; ++k)
     {
-        const int i = t + k * TP;
-        v[k]        = i == HN ? pack(HN, v[k], v[k]) : at(i - HN);
+        const int i  = tp + k * TP;
+        const bool h = k == E / 2 && tp == 0;
+        const cx<T> c = at(i - HN);
+        v[k]          = h ? pack(v[k], v[k], cconj(wn[HN])) : c;
     }
     lds_barrier();
-    run_pass<T, L, true, role_direct, prec_full, 0>(v, lds, tb, t, tw, K);
+    run_pass<T, L, true, role_direct, prec_full, 0>(v, lds, tb, fresh(t), tw, K);
     // v[k] = u[t + k TPT]; the upper half of the slots crosses to the lower half in mirror order
+    const int ts = fresh(t);
 #pragma unroll
     for (int k = E / 2; k < E; ++k)
-        at(t + k * TP - HN) = v[k];
+        at(ts + k * TP - HN) = v[k];
     lds_barrier();
     if (!valid)
         return;
 #pragma unroll
     for (int k = 0; k < E / 2; ++k)
     {
-        const int q    = t + k * TP;
+        const int q    = ts + k * TP;
         const cx<T> uq = v[k];
         const cx<T> um = at(HN - 1 - q); // u[N - 1 - q]
         quad<T> o4     = { uq.re, um.im, uq.im, um.re };
@@ -1323,11 +1328,9 @@ bool dct2_any(bool inverse, int L, const T* x, T* y, const cx<T>* tw, const cx<T
 // packed inputs of both slots i and N - i, and returns the latter the same way; the inverse transform
 // runs, and the reals are stored as plain pairs.
 template <typename T, int L>
-__global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_rfft_fwd(const cx<T>* __restrict__ x,
-                                                                            cx<T>* __restrict__ out, long long odist,
-                                                                            const cx<T>* __restrict__ tw,
-                                                                            const cx<T>* __restrict__ rtw, int fmt,
-                                                                            long long count)
+__global__ void __launch_bounds__((geom<T, L, role_direct>::BS), (geom<T, L, role_direct>::lds_waves))
+k_rfft_fwd(const cx<T>* __restrict__ x, cx<T>* __restrict__ out, long long odist, const cx<T>* __restrict__ tw,
+           const cx<T>* __restrict__ rtw, int fmt, long long count)
 {
     using G          = geom<T, L, role_direct>;
     using S          = typename G::S;
@@ -1351,75 +1354,66 @@ __global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_rfft_fwd(cons
 #pragma unroll
     for (int k = 0; k < E; ++k)
         v[k] = valid ? xb[k * TP] : cx<T>{ T(0), T(0) };
-    run_pass<T, L, false, role_direct, prec_full, 0>(v, lds, tb, t, tw, K);
+    run_pass<T, L, false, role_direct, prec_full, 0>(v, lds, tb, fresh(t), tw, K);
     cx<T>* o  = out + g * odist;
-    auto emit = [&](int i, cx<T> a, cx<T> c)
+    // X[0] (and X[N]) from Z[0]
+    auto emit_ends = [&](cx<T> a)
     {
-        if (!valid)
-            return;
-        if (i == 0)
+        if (fmt == 1)
         {
-            if (fmt == 1)
-            {
-                o[0] = { a.re + a.im, T(0) };
-                o[N] = { a.re - a.im, T(0) };
-            }
-            else
-                o[0] = { a.re + a.im, a.re - a.im };
-            return;
-        }
-        if (i == HN)
-        {
-            o[HN] = { a.re, -a.im };
-            return;
-        }
-        if (i < HN)
-        {
-            const cx<T> f1 = { a.re + c.re, a.im - c.im }; // a + conj c
-            const cx<T> f2 = { a.re - c.re, a.im + c.im }; // a - conj c
-            const cx<T> tt = cmul(f2, rtw[i]);
-            o[i]           = { T(0.5) * (f1.re + tt.re), T(0.5) * (f1.im + tt.im) };
+            o[0] = { a.re + a.im, T(0) };
+            o[N] = { a.re - a.im, T(0) };
         }
         else
-        {
-            // pair (k, N - k) with k = N - i: Z[k] = c, Z[N - k] = a
-            const cx<T> f1 = { c.re + a.re, c.im - a.im };
-            const cx<T> f2 = { c.re - a.re, c.im + a.im };
-            const cx<T> tt = cmul(f2, rtw[N - i]);
-            o[i]           = { T(0.5) * (f1.re - tt.re), -T(0.5) * (f1.im - tt.im) };
-        }
+            o[0] = { a.re + a.im, a.re - a.im };
+    };
+    // X[i] for 0 < i < N/2 (a = Z[i], c = Z[N-i])
+    auto emit_lo = [&](int i, cx<T> a, cx<T> c)
+    {
+        const cx<T> f1 = { a.re + c.re, a.im - c.im }; // a + conj c
+        const cx<T> f2 = { a.re - c.re, a.im + c.im }; // a - conj c
+        const cx<T> tt = cmul(f2, rtw[i]);
+        o[i]           = { T(0.5) * (f1.re + tt.re), T(0.5) * (f1.im + tt.im) };
+    };
+    // X[i] for N/2 < i < N: the pair (k, N - k) with k = N - i, Z[k] = c, Z[N - k] = a
+    auto emit_hi = [&](int i, cx<T> a, cx<T> c)
+    {
+        const cx<T> f1 = { c.re + a.re, c.im - a.im };
+        const cx<T> f2 = { c.re - a.re, c.im + a.im };
+        const cx<T> tt = cmul(f2, rtw[N - i]);
+        o[i]           = { T(0.5) * (f1.re - tt.re), -T(0.5) * (f1.im - tt.im) };
     };
     // the upper slots publish Z; each lower slot i reads Z[N-i] and writes the outputs of both i and
-    // N - i (the second run in reverse order, still coalesced); slot N/2 pairs with itself
+    // N - i (the second run in reverse order, still coalesced); slot N/2 pairs with itself. The
+    // indices come from a fresh copy of t, so none is held through the passes.
+    const int ts = fresh(t);
 #pragma unroll
     for (int k = E / 2; k < E; ++k)
-        at(t + k * TP - HN) = v[k];
+        at(ts + k * TP - HN) = v[k];
     lds_barrier();
+    if (!valid)
+        return;
 #pragma unroll
     for (int k = 0; k < E / 2; ++k)
     {
-        const int i = t + k * TP;
-        if (i == 0)
-            emit(0, v[k], v[k]);
+        const int i = ts + k * TP;
+        if (k == 0 && ts == 0)
+            emit_ends(v[k]);
         else
         {
             const cx<T> c = at(N - i - HN);
-            emit(i, v[k], c);
-            emit(N - i, c, v[k]);
+            emit_lo(i, v[k], c);
+            emit_hi(N - i, c, v[k]);
         }
     }
-#pragma unroll
-    for (int k = E / 2; k < E; ++k)
-        if (t + k * TP == HN)
-            emit(HN, v[k], v[k]);
+    if (ts == 0)
+        o[HN] = cconj(v[E / 2]); // X[N/2] = conj Z[N/2]
 }
 
 template <typename T, int L>
-__global__ void __launch_bounds__((geom<T, L, role_direct>::BS)) k_rfft_inv(const cx<T>* __restrict__ spec,
-                                                                            long long idist, cx<T>* __restrict__ y,
-                                                                            const cx<T>* __restrict__ tw,
-                                                                            const cx<T>* __restrict__ rtw, int fmt,
-                                                                            long long count)
+__global__ void __launch_bounds__((geom<T, L, role_direct>::BS), (geom<T, L, role_direct>::lds_waves))
+k_rfft_inv(const cx<T>* __restrict__ spec, long long idist, cx<T>* __restrict__ y, const cx<T>* __restrict__ tw,
+           const cx<T>* __restrict__ rtw, int fmt, long long count)
 {
     using G          = geom<T, L, role_direct>;
     using S          = typename G::S;

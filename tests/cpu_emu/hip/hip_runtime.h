@@ -81,6 +81,7 @@ EMU_SMEM_IN(kfr_amd::bt)
 #define __syncthreads() ::emu::sync()
 #define __builtin_amdgcn_s_barrier() ::emu::sync()
 #define __builtin_amdgcn_fence(...) ((void)0)
+#define __builtin_amdgcn_sched_barrier(...) ((void)0)
 typedef unsigned emu_u32x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 emu_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float emu_f32x16 __attribute__((ext_vector_type(16)));
