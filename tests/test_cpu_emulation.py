@@ -5,7 +5,7 @@ global access outside its buffer aborts the run; the spectra are checked against
 
 Cases: the generic four-step over factors 3, 5, 7 with workgroups that straddle batch items
 (f32 at 16 points per thread), the power-of-two four-step, the mixed-radix menu lengths, Bluestein, a
-real transform, a DCT, the one-kernel DCT-II/III and R2C/C2R; half-precision storage on emulated MFMA; and the f64
+real transform, a DCT, the one-kernel DCT-II/III and R2C/C2R, the R2C repack pass; half-precision storage on emulated MFMA; and the f64
 generic kernels built at 16 points per thread (the geometry an earlier build ran on the GPU, where
 lengths with radix-5 passes came out wrong) on those lengths.
 Set KFR_AMD_EMU_TSAN=1 to also run a ThreadSanitizer build (a missing barrier shows as a race), and
@@ -30,12 +30,12 @@ from tests.common import fp64_dct, fp64_rdft  # noqa: E402
 pytestmark = pytest.mark.skipif(not Path(emu_build.CXX).exists(), reason="ROCm clang++ not found")
 
 
-def _run(variant, tmp_path, args, x, out_dtype):
+def _run(variant, tmp_path, args, x, out_dtype, env=None):
     exe = emu_build.build(variant)
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     np.ascontiguousarray(x).tofile(fin)
     r = subprocess.run([str(exe)] + [str(a) for a in args] + [str(fin), str(fout)], capture_output=True, text=True,
-                       timeout=900)
+                       timeout=900, env=None if env is None else dict(os.environ, **env))
     assert r.returncode == 0, r.stderr[-4000:]
     return np.fromfile(fout, dtype=out_dtype)
 
@@ -119,6 +119,16 @@ def test_emulated_c2r_unpack_pass(n, batch, fmt, tmp_path):
     X = fp64_rdft(x.astype(np.float64), fmt).astype(np.complex64)
     y = _run("asan", tmp_path, ["real", "f32", n, fmt, batch, 1], X, np.float32).reshape(batch, n)
     assert oracle.rel_rms(y / n, x) <= 1e-6
+
+
+# R2C whose half does not fit one workgroup: the complex 2^16-point half transform, then the repack
+# pass over the half spectrum in global memory
+@pytest.mark.parametrize("fmt", [0, 1])
+def test_emulated_r2c_repack_pass(fmt, tmp_path):
+    n, batch = 1 << 17, 1
+    x = oracle.gaussian(76 + fmt, batch * n, np.float32).reshape(batch, n)
+    X = _run("asan", tmp_path, ["real", "f32", n, fmt, batch, 0], x, np.complex64).reshape(batch, -1)
+    assert oracle.rel_rms(X, fp64_rdft(x.astype(np.float64), fmt)) <= 1e-6
 
 
 # DCT-II and DCT-III in one kernel: f32 at 2^8 points per half (16 threads per transform, the shortest
