@@ -86,7 +86,7 @@ def traffic(fetch_db: str, write_db: str, probe_match: str, probe_bytes: int) ->
         rows.append((len(fv), fa, wa, (fa * sf + wa * sw) * 1024.0, kernel_key(name)))
     rows.sort(key=lambda r: -r[3] * r[0])
     for n, fa, wa, tb, key in rows:
-        out.append(f"| {n} | {fa:.1f} | {wa:.1f} | {tb:.0f} | `{key}` |")
+        out.append(f"| {n} | {fa:.1f} | {wa:.1f} | {tb:.0f} | `{_short(key)}` |")
     return "\n".join(out)
 
 
