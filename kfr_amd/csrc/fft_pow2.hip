@@ -413,11 +413,24 @@ __device__ __forceinline__ void lds_barrier()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// the thread whose positions feed the second butterfly of thread t in a mirrored last pass
+template <int MODE, int TP>
+__device__ __forceinline__ int mirror_lane(int t)
+{
+    return MODE == 1 ? TP - 1 - t : (TP - t) & (TP - 1);
+}
+
 // pass P of the schedule on the registers v (strided set in, strided set out after the exchange).
-// MIRROR0 (first pass of two butterflies per thread): the second butterfly of thread t is
-// 2 TPT-1-t instead of t + TPT, so that x[q] and x[N-1-q] are inputs of the same thread's two
-// butterflies (the mirrored input of the one-kernel DCT-II then needs no exchange).
-template <typename T, int L, bool INV, int ROLE, int PREC, int P, bool MIRROR0 = false>
+// MIRROR0 (first pass of an even number M of butterflies per thread): butterfly m >= M/2 of thread t
+// is TPT-1-t + m TPT instead of t + m TPT, so that x[q] and x[N-1-q] are inputs of the same thread's
+// butterflies m and M-1-m (the mirrored input of the one-kernel DCT-II then needs no exchange).
+// MIRRORL (last pass of two butterflies per thread): slot m + 2r holds X[j_m + 2r TPT] with j_0 = t and
+// j_1 = TPT + mirror(t), the exchange before it reading the odd slots from the positions of thread
+// mirror(t). mirror_rev (mirror(t) = TPT-1-t) gives one thread both X[q] and X[N-1-q]; mirror_neg
+// (mirror(t) = (TPT-t) mod TPT) both X[q] and X[N-q] (thread 0's butterflies 0 and TPT pair with
+// themselves).
+constexpr int mirror_none = 0, mirror_rev = 1, mirror_neg = 2;
+template <typename T, int L, bool INV, int ROLE, int PREC, int P, bool MIRROR0 = false, int MIRRORL = mirror_none>
 __device__ __forceinline__ void run_pass(regs<T, L, ROLE, PREC>& v, cx<T>* lds, int tb, int t, const cx<T>* tw,
                                          const mfma16& K)
 {
@@ -434,7 +447,9 @@ __device__ __forceinline__ void run_pass(regs<T, L, ROLE, PREC>& v, cx<T>* lds, 
 #pragma unroll
         for (int m = 0; m < M; ++m)
         {
-            const int j  = t + m * TP;
+            const int j  = MIRRORL != mirror_none && P + 1 == S::NP && m >= M / 2
+                               ? mirror_lane<MIRRORL, TP>(t) + m * TP
+                               : t + m * TP;
             const int kt = j & (NS - 1);
             cx<T> w[R];
             if constexpr (G::tw_pow && R >= 8)
@@ -500,7 +515,13 @@ __device__ __forceinline__ void run_pass(regs<T, L, ROLE, PREC>& v, cx<T>* lds, 
             lds_barrier();
 #pragma unroll
             for (int i = 0; i < E / H; ++i)
-                v[h * MH + i % MH + (i / MH) * M] = lds[laddr<ROLE, G::W, G::LD>(tb, t + (h * (E / H) + i) * TP - h * G::NH)];
+            {
+                // position t + k TPT feeds slot k of the next pass; a mirrored last pass takes its odd
+                // slots (its second butterfly) from position mirror(t) + k TPT
+                const int k  = h * (E / H) + i;
+                const int tk = MIRRORL != mirror_none && P + 2 == S::NP && (k & 1) ? mirror_lane<MIRRORL, TP>(t) : t;
+                v[h * MH + i % MH + (i / MH) * M] = lds[laddr<ROLE, G::W, G::LD>(tb, tk + k * TP - h * G::NH)];
+            }
             lds_barrier();
         }
         if constexpr (H > 1)
@@ -515,9 +536,12 @@ __device__ __forceinline__ void run_pass(regs<T, L, ROLE, PREC>& v, cx<T>* lds, 
             for (int k = 0; k < E; ++k)
                 v[k] = u[k];
         }
-        run_pass<T, L, INV, ROLE, PREC, P + 1>(v, lds, tb, t, tw, K);
+        run_pass<T, L, INV, ROLE, PREC, P + 1, false, MIRRORL>(v, lds, tb, t, tw, K);
     }
-    static_assert(!MIRROR0 || (P == 0 && NS == 1 && M == 2 && P + 1 < S::NP), "mirrored first pass: two butterflies");
+    static_assert(!MIRROR0 || (P == 0 && NS == 1 && M % 2 == 0 && P + 1 < S::NP),
+                  "mirrored first pass: an even number of butterflies");
+    static_assert(MIRRORL == mirror_none || (S::NP >= 2 && E / (1 << S::lr(S::NP - 1)) == 2),
+                  "mirrored last pass: two butterflies");
 }
 
 // power-of-two scale 2^k of a value array whose largest |re|, |im| is mx: the scaled values stay below
@@ -1021,9 +1045,11 @@ inline int log2_exact(size_t n)
 // the even and odd samples (from Z[k] and Z[N-k]); y[k] = Re(V[k] W_4n^k), y[n-k] = -Im(V[k] W_4n^k),
 // y[N] = Re((Re Z0 - Im Z0) W_4n^N). Thread t holds z[t + k TPT]: slots k < E/2 take z[q] straight
 // from one coalesced load of x[4q..4q+3], whose other half z[N-1-q] is slot E-1-k of thread TPT-1-t and
-// crosses through LDS (no exchange when the first pass has two butterflies per thread: they are
-// assigned in mirror pairs); after the passes one LDS round of N/2 values brings Z[N-i] to the slot i
-// < N/2, which writes the outputs of both i and N-i.
+// crosses through LDS, unless the first pass has several butterflies per thread: they are then
+// assigned in mirror pairs and each thread loads all of its inputs. After the passes, with two
+// butterflies per thread in the last pass, those are assigned so that one thread holds Z[i] and Z[N-i]
+// and writes the outputs of both; otherwise one LDS round of N/2 values brings Z[N-i] to the slot
+// i < N/2, which writes the outputs of both i and N-i.
 template <typename T>
 struct alignas(4 * sizeof(T)) quad
 {
@@ -1055,30 +1081,31 @@ k_dct2(const T* __restrict__ x, T* __restrict__ y, const cx<T>* __restrict__ tw,
     auto at           = [&](int i) -> cx<T>& { return lds[laddr<role_direct, W, G::LD>(tb, i)]; };
     cx<T> v[E];
     constexpr int M0 = E >> S::lr(0); // butterflies per thread in the first pass
-    if constexpr (M0 == 2 && S::NP > 1)
+    // two butterflies per thread in the last pass: assigned in pairs whose outputs are Z[i] and Z[N-i]
+    constexpr int ML = G::EX == 2 && S::NP >= 2 ? mirror_neg : mirror_none;
+    if constexpr (M0 >= 2 && S::NP > 1)
     {
-        // The first pass's butterflies of thread t are j0 = t and j1 = 2 TPT-1-t (inputs j + r N/R0,
-        // N/R0 = 2 TPT; slot 2r holds input r of j0, slot 2r+1 input r of j1). For r < R0/2, quad
-        // q = j0 + r N/R0 holds z[q] (input r of j0) and z[N-1-q] (input R0-1-r of j1), and quad
-        // q = j1 + r N/R0 holds z[q] (input r of j1) and z[N-1-q] (input R0-1-r of j0): the thread
-        // loads all of its inputs itself, no exchange
-        constexpr int R0 = E / 2;
+        // The first pass's butterflies of thread t are j_m = t + m TPT for m < M0/2 and
+        // j_m = TPT-1-t + m TPT for m >= M0/2, so j_m + j_(M0-1-m) = N/R0 - 1 (inputs j + r N/R0 in
+        // slot m + r M0). For r < R0/2 quad q = j_m + r N/R0 holds z[q] (input r of j_m) and
+        // z[N-1-q] (input R0-1-r of j_(M0-1-m)): the thread loads all of its inputs itself, no exchange
+        constexpr int R0  = E / M0;
         const quad<T>* xq = reinterpret_cast<const quad<T>*>(x + g * n);
 #pragma unroll
-        for (int r = 0; r < R0 / 2; ++r)
+        for (int m = 0; m < M0; ++m)
         {
-            quad<T> qa = { T(0), T(0), T(0), T(0) }, qb = qa;
-            if (valid)
+            const int jm = m < M0 / 2 ? t + m * TP : TP - 1 - t + m * TP;
+#pragma unroll
+            for (int r = 0; r < R0 / 2; ++r)
             {
-                qa = xq[t + r * (2 * TP)];
-                qb = xq[2 * TP - 1 - t + r * (2 * TP)];
+                quad<T> q4 = { T(0), T(0), T(0), T(0) };
+                if (valid)
+                    q4 = xq[jm + r * (M0 * TP)];
+                v[m + r * M0]                       = { q4.a, q4.c };
+                v[(M0 - 1 - m) + (R0 - 1 - r) * M0] = { q4.d, q4.b };
             }
-            v[2 * r]                = { qa.a, qa.c };
-            v[1 + 2 * (R0 - 1 - r)] = { qa.d, qa.b };
-            v[1 + 2 * r]            = { qb.a, qb.c };
-            v[2 * (R0 - 1 - r)]     = { qb.d, qb.b };
         }
-        run_pass<T, L, false, role_direct, prec_full, 0, true>(v, lds, tb, fresh(t), tw, K);
+        run_pass<T, L, false, role_direct, prec_full, 0, true, ML>(v, lds, tb, fresh(t), tw, K);
     }
     else
     {
@@ -1096,7 +1123,7 @@ k_dct2(const T* __restrict__ x, T* __restrict__ y, const cx<T>* __restrict__ tw,
         for (int k = E / 2; k < E; ++k)
             v[k] = at((k - E / 2) * TP + t);
         lds_barrier();
-        run_pass<T, L, false, role_direct, prec_full, 0>(v, lds, tb, fresh(t), tw, K);
+        run_pass<T, L, false, role_direct, prec_full, 0, false, ML>(v, lds, tb, fresh(t), tw, K);
     }
     // v[k] = Z[t + k TPT]. Output pair i, n - i needs Z[i] and Z[N - i] (Z[0] for i = 0).
     T* yb     = y + g * n;
@@ -1119,38 +1146,83 @@ k_dct2(const T* __restrict__ x, T* __restrict__ y, const cx<T>* __restrict__ tw,
         yb[i]          = V.re * w.re - V.im * w.im;
         yb[n - i]      = -(V.re * w.im + V.im * w.re);
     };
-    // the upper slots publish Z; each lower slot i reads Z[N-i] and writes the outputs of both i and
-    // N - i; slot N/2 pairs with itself. The indices come from a fresh copy of t, so none is held
-    // through the passes.
-    const int ts = fresh(t);
-#pragma unroll
-    for (int k = E / 2; k < E; ++k)
-        at(ts + k * TP - HN) = v[k];
-    lds_barrier();
-    if (!valid)
-        return;
-#pragma unroll
-    for (int k = 0; k < E / 2; ++k)
+    if constexpr (ML == mirror_neg)
     {
-        const int i = ts + k * TP;
-        if (k == 0 && ts == 0)
-            emit0(v[k]);
+        // slot m + 2r holds Z[j_m + 2r TPT] with j_0 = t, j_1 = 2 TPT - t (thread 0: j_1 = TPT), so
+        // Z[N-i] of i = t + 2r TPT is slot 1 + 2(R-1-r); thread 0's butterflies pair with themselves
+        constexpr int R = E / 2;
+        if (!valid)
+            return;
+        const int ts = fresh(t);
+        if (ts != 0)
+        {
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+            {
+                const int i   = ts + 2 * r * TP;
+                const cx<T> a = v[2 * r], c = v[1 + 2 * (R - 1 - r)];
+                emit(i, a, c);
+                emit(N - i, c, a);
+            }
+        }
         else
         {
-            const cx<T> c = at(N - i - HN);
-            emit(i, v[k], c);
-            emit(N - i, c, v[k]);
+            emit0(v[0]);
+#pragma unroll
+            for (int r = 1; r < R / 2; ++r)
+            {
+                const int i = 2 * r * TP;
+                emit(i, v[2 * r], v[2 * (R - r)]);
+                emit(N - i, v[2 * (R - r)], v[2 * r]);
+            }
+            emit(HN, v[R], v[R]);
+#pragma unroll
+            for (int r = 0; r < R / 2; ++r)
+            {
+                const int i = TP + 2 * r * TP;
+                emit(i, v[1 + 2 * r], v[1 + 2 * (R - 1 - r)]);
+                emit(N - i, v[1 + 2 * (R - 1 - r)], v[1 + 2 * r]);
+            }
         }
+        return;
     }
-    if (ts == 0)
-        emit(HN, v[E / 2], v[E / 2]);
+    else
+    {
+        // the upper slots publish Z; each lower slot i reads Z[N-i] and writes the outputs of both i
+        // and N - i; slot N/2 pairs with itself. The indices come from a fresh copy of t, so none is
+        // held through the passes.
+        const int ts = fresh(t);
+#pragma unroll
+        for (int k = E / 2; k < E; ++k)
+            at(ts + k * TP - HN) = v[k];
+        lds_barrier();
+        if (!valid)
+            return;
+#pragma unroll
+        for (int k = 0; k < E / 2; ++k)
+        {
+            const int i = ts + k * TP;
+            if (k == 0 && ts == 0)
+                emit0(v[k]);
+            else
+            {
+                const cx<T> c = at(N - i - HN);
+                emit(i, v[k], c);
+                emit(N - i, c, v[k]);
+            }
+        }
+        if (ts == 0)
+            emit(HN, v[E / 2], v[E / 2]);
+    }
 }
 
 // DCT-III (dct_plan's inverse, unscaled) in one kernel, the mirror image of k_dct2: with V[k] = y'[k] W_4n^k
 // (y'[0] = y[0]/2) and H[k] = (conj V[k] + V[n-k])/2, slot i forms
 // Z'[i] = (H[i] + conj H[N-i]) + i (H[i] - conj H[N-i]) conj(W_n^i), the inverse N-point transform runs, and
-// x[4q..4q+3] = (u[q].re, u[N-1-q].im, u[q].im, u[N-1-q].re) leaves as one 16-byte store per q < N/2,
-// u[N-1-q] crossing through LDS from slot E-1-k of thread TPT-1-t.
+// x[4q..4q+3] = (u[q].re, u[N-1-q].im, u[q].im, u[N-1-q].re) leaves as one 16-byte store per q < N/2.
+// With two butterflies per thread in the last pass they are assigned in mirror pairs and each thread
+// holds both u[q] and u[N-1-q]; otherwise u[N-1-q] crosses through LDS from slot E-1-k of thread
+// TPT-1-t.
 template <typename T, int L>
 __global__ void __launch_bounds__((geom<T, L, role_direct>::BS), (geom<T, L, role_direct>::lds_waves))
 k_dct3(const T* __restrict__ y, T* __restrict__ x, const cx<T>* __restrict__ tw, const cx<T>* __restrict__ wn,
@@ -1242,23 +1314,45 @@ k_dct3(const T* __restrict__ y, T* __restrict__ x, const cx<T>* __restrict__ tw,
         v[k]          = h ? pack(v[k], v[k], cconj(wn[HN])) : c;
     }
     lds_barrier();
-    run_pass<T, L, true, role_direct, prec_full, 0>(v, lds, tb, fresh(t), tw, K);
-    // v[k] = u[t + k TPT]; the upper half of the slots crosses to the lower half in mirror order
-    const int ts = fresh(t);
-#pragma unroll
-    for (int k = E / 2; k < E; ++k)
-        at(ts + k * TP - HN) = v[k];
-    lds_barrier();
-    if (!valid)
-        return;
-#pragma unroll
-    for (int k = 0; k < E / 2; ++k)
+    quad<T>* xq = reinterpret_cast<quad<T>*>(x + g * n);
+    if constexpr (G::EX == 2 && S::NP >= 2)
     {
-        const int q    = ts + k * TP;
-        const cx<T> uq = v[k];
-        const cx<T> um = at(HN - 1 - q); // u[N - 1 - q]
-        quad<T> o4     = { uq.re, um.im, uq.im, um.re };
-        reinterpret_cast<quad<T>*>(x + g * n)[q] = o4;
+        // two butterflies per thread in the last pass, assigned in mirror pairs: slot m + 2r holds
+        // u[j_m + 2r TPT] (j_0 = t, j_1 = 2 TPT-1-t), and u[N-1-q] of q = j_m + 2r TPT is output R-1-r
+        // of the other butterfly, so every quad leaves from registers
+        constexpr int R = E / 2;
+        run_pass<T, L, true, role_direct, prec_full, 0, false, mirror_rev>(v, lds, tb, fresh(t), tw, K);
+        if (!valid)
+            return;
+        const int ts = fresh(t);
+#pragma unroll
+        for (int r = 0; r < R / 2; ++r)
+        {
+            const cx<T> a0 = v[2 * r], a1 = v[1 + 2 * (R - 1 - r)];
+            xq[ts + 2 * r * TP] = quad<T>{ a0.re, a1.im, a0.im, a1.re };
+            const cx<T> b0 = v[1 + 2 * r], b1 = v[2 * (R - 1 - r)];
+            xq[2 * TP - 1 - ts + 2 * r * TP] = quad<T>{ b0.re, b1.im, b0.im, b1.re };
+        }
+    }
+    else
+    {
+        run_pass<T, L, true, role_direct, prec_full, 0>(v, lds, tb, fresh(t), tw, K);
+        // v[k] = u[t + k TPT]; the upper half of the slots crosses to the lower half in mirror order
+        const int ts = fresh(t);
+#pragma unroll
+        for (int k = E / 2; k < E; ++k)
+            at(ts + k * TP - HN) = v[k];
+        lds_barrier();
+        if (!valid)
+            return;
+#pragma unroll
+        for (int k = 0; k < E / 2; ++k)
+        {
+            const int q    = ts + k * TP;
+            const cx<T> uq = v[k];
+            const cx<T> um = at(HN - 1 - q); // u[N - 1 - q]
+            xq[q]          = quad<T>{ uq.re, um.im, uq.im, um.re };
+        }
     }
 }
 
@@ -1933,8 +2027,8 @@ pow2_fft<T>::pow2_fft(size_t n, hipStream_t s, int prec) : p(new impl)
     // 2^10..2^12 points near a third, so uneven splits often win
     if (prec == p2::prec_full)
     {
-        static constexpr int best_f32[] = { 8, 8, 9, 10, 8, 8, 9, 10, 11 };  // L = 15..23
-        static constexpr int best_f64[] = { 7, 9, 9, 8, 9, 8, 9, 10, 10 };  // L = 14..22
+        static constexpr int best_f32[] = { 8, 8, 9, 9, 8, 8, 9, 10, 11 };  // L = 15..23
+        static constexpr int best_f64[] = { 7, 9, 7, 8, 9, 8, 9, 10, 11 };  // L = 14..22
         if (sizeof(T) == 4 && L >= 15 && L <= 23)
             p->L2 = best_f32[L - 15];
         if (sizeof(T) == 8 && L >= 14 && L <= 22)

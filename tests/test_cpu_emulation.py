@@ -132,10 +132,11 @@ def test_emulated_r2c_repack_pass(fmt, tmp_path):
 
 
 # DCT-II and DCT-III in one kernel: f32 at 2^8 points per half (16 threads per transform, the shortest
-# fused length) and 2^15 (two-round LDS exchange), f64 at 2^13 and 2^14; 128 points f32 stays on the
+# fused length), 2^10 (mirrored first pass, LDS partner exchange) and 2^13, 2^15 (two-round LDS
+# exchange, mirrored first and last passes), f64 at 2^13 and 2^14; 128 points f32 stays on the
 # pack/FFT/post path
-@pytest.mark.parametrize("t,m,batch", [("f32", 512, 5), ("f32", 256, 3), ("f32", 1 << 16, 1), ("f64", 1 << 14, 2),
-                                       ("f64", 1 << 15, 1)])
+@pytest.mark.parametrize("t,m,batch", [("f32", 512, 5), ("f32", 256, 3), ("f32", 2048, 3), ("f32", 1 << 14, 2),
+                                       ("f32", 1 << 16, 1), ("f64", 1 << 14, 2), ("f64", 1 << 15, 1)])
 def test_emulated_fused_dct(t, m, batch, tmp_path):
     dt = np.float32 if t == "f32" else np.float64
     d = oracle.gaussian(73 + m % 7, batch * m, dt).reshape(batch, m)
