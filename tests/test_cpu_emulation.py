@@ -95,9 +95,10 @@ def test_emulated_real_and_dct(tmp_path):
         assert oracle.rel_rms(y, fp64_dct(d.astype(np.float64), inverse)) <= 2e-6
 
 
-# R2C and C2R in one kernel for power-of-two halves that fit one workgroup (f32 2^8 and 2^15 halves,
-# the two-round partner exchange at 2^15; f64 2^14), both pack formats
-@pytest.mark.parametrize("t,n,batch", [("f32", 512, 5), ("f32", 1 << 16, 2), ("f64", 1 << 15, 2)])
+# R2C and C2R in one kernel for power-of-two halves that fit one workgroup (f32 2^8, 2^10, 2^13 and
+# 2^15 halves, the last two with two-round LDS exchanges; f64 2^14), both pack formats
+@pytest.mark.parametrize("t,n,batch", [("f32", 512, 5), ("f32", 2048, 3), ("f32", 1 << 14, 2), ("f32", 1 << 16, 2),
+                                       ("f64", 1 << 15, 2)])
 def test_emulated_real_one_kernel(t, n, batch, tmp_path):
     dt = np.float32 if t == "f32" else np.float64
     cdt = np.complex64 if t == "f32" else np.complex128
