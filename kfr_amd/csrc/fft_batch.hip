@@ -168,7 +168,10 @@ __global__ void __launch_bounds__(block) __attribute__((amdgpu_waves_per_eu(4)))
             cx<T> v = c.at(i);
             if (D.bluestein)
                 v = cmul(cconj(v), ch[i]);
-            out[off + i] = inv ? cconj(v) : v;
+            v = inv ? cconj(v) : v;
+            // nontemporal: the outputs are not read again by this launch (measured 4% faster)
+            using V2 = T __attribute__((ext_vector_type(2)));
+            __builtin_nontemporal_store(V2{ v.re, v.im }, reinterpret_cast<V2*>(out + off + i));
         }
         if (threadIdx.x == 0)
             s_item = (int)next;
