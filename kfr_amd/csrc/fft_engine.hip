@@ -439,7 +439,9 @@ void gpu_dft<T>::progressive_unit(int u, void* data, void* mid, bool inverse, hi
 // spectrum is repacked (to_fmt / from_fmt, src/dft/fft-impl.hpp:1799-1896) with the twiddles
 // tw[k] = exp(-i (2 pi k/n + pi/2)) = W_{4n}^{4k+n}.
 
-template <typename T>
+// NT: nontemporal stores (the repack pass over spectra in global memory: R2C 2^24 x 16 and
+// 30,720,000 x 8 measured 1.5-2.7% faster with them)
+template <typename T, bool NT = false>
 __device__ __forceinline__ void repack_pair(cx<T>* o, const cx<T>* __restrict__ rtw, long long csize, long long k,
                                             cx<T> zk, cx<T> znk)
 {
@@ -448,8 +450,19 @@ __device__ __forceinline__ void repack_pair(cx<T>* o, const cx<T>* __restrict__ 
     const cx<T> f1   = cadd(zk, fpnk);
     const cx<T> f2   = csub(zk, fpnk);
     const cx<T> t    = cmul(f2, rtw[k]);
-    o[k]             = cscale(cadd(f1, t), T(0.5));
-    o[csize - k]     = cconj(cscale(csub(f1, t), T(0.5)));
+    const cx<T> a    = cscale(cadd(f1, t), T(0.5));
+    const cx<T> b    = cconj(cscale(csub(f1, t), T(0.5)));
+    if constexpr (NT)
+    {
+        using V2 = T __attribute__((ext_vector_type(2)));
+        __builtin_nontemporal_store(V2{ a.re, a.im }, reinterpret_cast<V2*>(o + k));
+        __builtin_nontemporal_store(V2{ b.re, b.im }, reinterpret_cast<V2*>(o + (csize - k)));
+    }
+    else
+    {
+        o[k]         = a;
+        o[csize - k] = b;
+    }
 }
 template <typename T>
 __device__ __forceinline__ void repack_dc(cx<T>* o, long long csize, int fmt, cx<T> dc)
@@ -483,7 +496,7 @@ __global__ void k_real_repack(cx<T>* out, long long odist, const cx<T>* Z, long 
             o[csize / 2] = cconj(z[csize / 2]);
     }
     else
-        repack_pair(o, rtw, csize, k, z[k], z[csize - k]);
+        repack_pair<T, true>(o, rtw, csize, k, z[k], z[csize - k]);
 }
 
 // fused forward real transform for LDS-resident half lengths: the FFT result is repacked from LDS
