@@ -37,7 +37,8 @@ def test_real_dft(n, fmt, dt):
     assert e2 <= bound * 2, e2
 
 
-@pytest.mark.parametrize("n,dt", [(512, np.float32), (1 << 16, np.float32), (256, np.float64), (1 << 15, np.float64)])
+@pytest.mark.parametrize("n,dt", [(512, np.float32), (2048, np.float32), (1 << 14, np.float32), (1 << 16, np.float32),
+                                  (256, np.float64), (2048, np.float64), (1 << 15, np.float64)])
 @pytest.mark.parametrize("fmt", [0, 1])
 def test_real_one_kernel_ragged_batch(n, dt, fmt):
     """R2C and C2R in one kernel (power-of-two halves 2^8..2^15 f32, 2^7..2^14 f64) over a batch that
@@ -103,10 +104,14 @@ def test_dct(n, dt):
         assert e <= max(2e-6 if dt == np.float32 else 1e-13, tol_kfr(n, dt)), (n, inv, e)
 
 
-@pytest.mark.parametrize("n,dt", [(512, np.float32), (2048, np.float32), (256, np.float64), (16384, np.float64)])
+@pytest.mark.parametrize("n,dt", [(512, np.float32), (1024, np.float32), (2048, np.float32), (8192, np.float32),
+                                  (16384, np.float32), (32768, np.float32), (65536, np.float32), (256, np.float64),
+                                  (512, np.float64), (1024, np.float64), (2048, np.float64), (8192, np.float64),
+                                  (16384, np.float64), (32768, np.float64)])
 def test_dct2_one_kernel_ragged_batch(n, dt):
-    """the one-kernel DCT-II and DCT-III (half lengths 2^8..2^15 in f32, 2^7..2^14 in f64) over a batch
-    that leaves the last workgroup part-filled; every transform against FP64"""
+    """the one-kernel DCT-II and DCT-III (half lengths 2^8..2^15 in f32, 2^7..2^14 in f64: with and
+    without the mirrored first and last passes) over a batch that leaves the last workgroup
+    part-filled; every transform against FP64"""
     torch = pytest.importorskip("torch")
     b = 37
     x = oracle.gaussian(77 + n, n * b, dt)
