@@ -145,18 +145,19 @@ struct geom
     }
     static constexpr int EX = ex();
     static constexpr int H  = EX; // LDS exchange rounds
-    // pass twiddles built from powers (measured faster nearly everywhere; the two kernels below lose
-    // a wave per SIMD to the extra registers)
-    static constexpr bool tw_pow = !(ROLE == role_direct && sizeof(T) == 4 && L >= 14) &&
-                                   !(ROLE == role_row && sizeof(T) == 8 && L == 10);
+    // pass twiddles built from powers (measured faster nearly everywhere; the f64 2^10 row kernel loses
+    // a wave per SIMD to the extra registers). The one-workgroup kernels are compiled for the waves their
+    // LDS allows (lds_waves), which keeps the f32 2^14 and 2^15 ones at 128 VGPRs with them.
+    static constexpr bool tw_pow = !(ROLE == role_row && sizeof(T) == 8 && L == 10);
     using S                 = shape<T, L, EX>;
     static constexpr int BS = S::TPT * W;
     static constexpr int NH = S::N / H; // positions per exchange round
     static constexpr int LD = ROLE == role_row ? NH + 1 : NH; // LDS distance between transforms
     static constexpr int logical = ROLE == role_row ? W * LD : W * NH;
     static constexpr size_t lds_bytes = (size_t)(logical + logical / 16 + 1) * sizeof(cx<T>);
-    // waves per SIMD that the workgroups' LDS lets one CU hold, up to 4 (128 VGPRs): the one-kernel
-    // real and DCT kernels are compiled for it so that their registers do not cost a workgroup per CU
+    // waves per SIMD that the workgroups' LDS lets one CU hold, up to 4 (128 VGPRs): the one-workgroup
+    // kernels (plain, real and DCT) are compiled for it so that their registers do not cost a
+    // workgroup per CU
     static constexpr int lds_waves = std::min<int>(4, (int)(163840 / lds_bytes) * (BS / 64) / 4);
     // short transforms (under 16 threads each) load and store through LDS in whole-workgroup runs
     static constexpr bool staged = ROLE == role_direct && S::TPT < 16;
@@ -587,7 +588,8 @@ __device__ __forceinline__ int workgroup_scale(const cx<T> (&v)[E], cx<T>* lds)
 }
 
 template <typename T, int L, bool INV, int ROLE, int PREC>
-__global__ void __launch_bounds__((geom<T, L, ROLE, PREC>::BS)) k_pow2(args<T> a)
+__global__ void __launch_bounds__((geom<T, L, ROLE, PREC>::BS),
+                                  (ROLE == role_direct ? geom<T, L, ROLE, PREC>::lds_waves : 1)) k_pow2(args<T> a)
 {
     using G          = geom<T, L, ROLE, PREC>;
     using S          = typename G::S;
