@@ -117,8 +117,12 @@ struct geom
             return S1::TPT >= 256 ? 1 : 256 / S1::TPT;
         // full-precision rows that would exceed the budget exchange in two rounds (ex() below), holding
         // half of each row in LDS at a time: twice the rows per workgroup and twice the run length of
-        // the transposed store (measured 2-6% faster for rows of 2^11..2^13 points)
-        constexpr int grow = ROLE == role_row && PREC == 0 && S1::LE >= log_e<T> ? 2 : 1;
+        // the transposed store (measured 2-6% faster for rows of 2^11..2^13 points); the same for the
+        // columns of which the budget holds fewer than 16 (f32 2^23 2.5%, f64 2^22 and 2^23 4-7% faster)
+        constexpr int grow = (ROLE == role_row || (ROLE == role_col && 16 * S1::N > lds_budget<T>)) && PREC == 0 &&
+                                     S1::LE >= log_e<T>
+                                 ? 2
+                                 : 1;
         // half-precision passes hold 32 butterflies: 128-byte runs of 4-byte values
         int w = ROLE == role_pass && PREC != 0 ? 32 : 16;
         while (w > 1 && (w * S1::N > grow * lds_budget<T> || w * S1::TPT > grow * 1024))
@@ -135,9 +139,11 @@ struct geom
     // threads and half the LDS, so two workgroups share a CU.
     static constexpr int ex()
     {
-        // measured: column kernels (whose strided loads, twiddles and stores hold the most registers)
-        // and the half-precision kernels (MFMA operands) run faster with the full workgroup
-        if (ROLE == role_pass || ROLE == role_conv || ROLE == role_col || PREC != 0 || S1::LE < log_e<T>)
+        // measured: column kernels of which the budget holds 16 columns (their strided loads, twiddles
+        // and stores hold the most registers) and the half-precision kernels (MFMA operands) run faster
+        // with the full workgroup
+        if (ROLE == role_pass || ROLE == role_conv || PREC != 0 || S1::LE < log_e<T> ||
+            (ROLE == role_col && 16 * S1::N <= lds_budget<T>))
             return 1;
         if (ROLE == role_direct || ROLE == role_block) // measured: f64 from 128 threads, f32 from 512
             return S1::TPT >= (sizeof(T) == 8 ? 128 : 512) ? 2 : 1;
