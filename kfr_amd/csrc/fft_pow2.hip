@@ -2060,8 +2060,8 @@ pow2_fft<T>::pow2_fft(size_t n, hipStream_t s, int prec) : p(new impl)
     // 2^10..2^12 points near a third, so uneven splits often win
     if (prec == p2::prec_full)
     {
-        static constexpr int best_f32[] = { 8, 8, 9, 9, 8, 8, 9, 10, 11 };  // L = 15..23
-        static constexpr int best_f64[] = { 7, 9, 7, 8, 9, 8, 9, 10, 11 };  // L = 14..22
+        static constexpr int best_f32[] = { 8, 8, 9, 9, 8, 8, 9, 11, 11 };  // L = 15..23
+        static constexpr int best_f64[] = { 7, 9, 7, 8, 9, 8, 9, 11, 11 };  // L = 14..22
         if (sizeof(T) == 4 && L >= 15 && L <= 23)
             p->L2 = best_f32[L - 15];
         if (sizeof(T) == 8 && L >= 14 && L <= 22)
