@@ -993,15 +993,16 @@ void launch_any(bool inv, int L, const args<T>& a, long long count, hipStream_t 
         launch_len<T, false, ROLE, PREC, LO, HI>(L, a, count, s);
 }
 
-// columns per workgroup of the column kernel of 2^L points (the row kernel's scale grouping)
-template <typename T, int LO, int HI>
+// columns per workgroup of the column kernel of 2^L points at precision PREC (the row kernel's scale
+// grouping; the full-precision and half-precision column kernels differ in width)
+template <typename T, int PREC, int LO, int HI>
 int col_width(int L)
 {
     if constexpr (LO <= HI)
     {
         if (L == LO)
-            return geom<T, LO, role_col>::W;
-        return col_width<T, LO + 1, HI>(L);
+            return geom<T, LO, role_col, PREC>::W;
+        return col_width<T, PREC, LO + 1, HI>(L);
     }
     else
         return 1;
@@ -2088,7 +2089,9 @@ pow2_fft<T>::pow2_fft(size_t n, hipStream_t s, int prec) : p(new impl)
     p2::gen_pass_tables<T>(p->L1, t + p->o_row, s);
     gen_twiddles(t + p->o_ta, nTA, (long long)n, nTB, 0, tw_linear, 0, s);
     gen_twiddles(t + p->o_tb, nTB, (long long)n, 1, 0, tw_linear, 0, s);
-    int w = p2::col_width<T, p2::min_split_log, p2::max_log<T>>(p->L2);
+    // binary16 and bfloat16 storage share one geometry
+    int w = prec == p2::prec_full ? p2::col_width<T, p2::prec_full, p2::min_split_log, p2::max_log<T>>(p->L2)
+                                  : p2::col_width<T, p2::prec_bf16, p2::min_split_log, p2::max_log<T>>(p->L2);
     while ((1 << p->gshift) < w)
         ++p->gshift;
     // rounds of at most 1 GiB of intermediate: few launches and round boundaries (measured 4-16% faster
