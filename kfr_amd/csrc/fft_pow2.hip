@@ -151,10 +151,6 @@ struct geom
     }
     static constexpr int EX = ex();
     static constexpr int H  = EX; // LDS exchange rounds
-    // pass twiddles built from powers (measured faster nearly everywhere; the f64 2^10 row kernel loses
-    // a wave per SIMD to the extra registers). The one-workgroup kernels are compiled for the waves their
-    // LDS allows (lds_waves), which keeps the f32 2^14 and 2^15 ones at 128 VGPRs with them.
-    static constexpr bool tw_pow = !(ROLE == role_row && sizeof(T) == 8 && L == 10);
     using S                 = shape<T, L, EX>;
     static constexpr int BS = S::TPT * W;
     static constexpr int NH = S::N / H; // positions per exchange round
@@ -162,8 +158,8 @@ struct geom
     static constexpr int logical = ROLE == role_row ? W * LD : W * NH;
     static constexpr size_t lds_bytes = (size_t)(logical + logical / 16 + 1) * sizeof(cx<T>);
     // waves per SIMD that the workgroups' LDS lets one CU hold, up to 4 (128 VGPRs): the one-workgroup
-    // kernels (plain, real and DCT) are compiled for it so that their registers do not cost a
-    // workgroup per CU
+    // kernels (plain, real and DCT) and the row kernels are compiled for it so that their registers do
+    // not cost a workgroup per CU
     static constexpr int lds_waves = std::min<int>(4, (int)(163840 / lds_bytes) * (BS / 64) / 4);
     // short transforms (under 16 threads each) load and store through LDS in whole-workgroup runs
     static constexpr bool staged = ROLE == role_direct && S::TPT < 16;
@@ -467,10 +463,13 @@ __device__ __forceinline__ void run_pass(regs<T, L, ROLE, PREC>& v, cx<T>* lds, 
                                : t + m * TP;
             const int kt = j & (NS - 1);
             cx<T> w[R];
-            if constexpr (G::tw_pow && R >= 8)
+            if constexpr (R >= 8)
             {
                 // W^(kt*r) from the row's entries at r = 2^b, one complex product per value: fewer
-                // loads than the whole row
+                // loads than the whole row (measured faster everywhere once the one-workgroup and row
+                // kernels are compiled for the waves their LDS allows: the f32 2^14 and 2^15
+                // one-workgroup kernels and the f64 2^10 row kernel had taken a wave per SIMD more
+                // registers and kept whole-row loads, 5-14% slower)
                 const cx<T>* row = tw + S::twoff(P) + kt * R;
                 w[0]             = { T(1), T(0) };
 #pragma unroll
@@ -595,7 +594,8 @@ __device__ __forceinline__ int workgroup_scale(const cx<T> (&v)[E], cx<T>* lds)
 
 template <typename T, int L, bool INV, int ROLE, int PREC>
 __global__ void __launch_bounds__((geom<T, L, ROLE, PREC>::BS),
-                                  (ROLE == role_direct ? geom<T, L, ROLE, PREC>::lds_waves : 1)) k_pow2(args<T> a)
+                                  (ROLE == role_direct || ROLE == role_row ? geom<T, L, ROLE, PREC>::lds_waves : 1))
+    k_pow2(args<T> a)
 {
     using G          = geom<T, L, ROLE, PREC>;
     using S          = typename G::S;
